@@ -1,0 +1,263 @@
+/*
+ * tdbp.h -- C ABI of the MI355X time-domain backprojection (TDBP) engine.
+ *
+ * Library: srg_amd/libsrg_tdbp.so (HIP, gfx950).  All entry points are
+ * extern "C" with plain pointers and sizes.  "d_" arguments are device
+ * pointers (HBM), everything else is host memory.  Streams are hipStream_t
+ * passed as void* (NULL = the null stream).
+ *
+ * Three layers:
+ *   1. The drop-in: azimuth_compress_, the symbol the reference Fortran
+ *      burst driver calls (by-reference arguments, host buffers, fds).
+ *   2. The operator API: the reference's device stages (setxyz, the aperture
+ *      fit, setazoff, pixelint) on device buffers, plus the fused burst
+ *      backprojection in FP64 ("plain") or mixed precision.
+ *   3. The batch API: many uneven bursts in one persistent launch with a
+ *      device work queue, captured in a hipGraph.
+ * Errors: every int-returning call returns a tdbp_status; the message of the
+ * last failure on the calling thread is tdbp_last_error().
+ */
+#ifndef SRG_TDBP_H
+#define SRG_TDBP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+  TDBP_OK = 0,
+  TDBP_ERR_ARG = 1,     /* invalid argument or geometry            */
+  TDBP_ERR_HIP = 2,     /* HIP runtime error                        */
+  TDBP_ERR_NODEV = 3,   /* no usable gfx950 device                  */
+  TDBP_ERR_IO = 4,      /* DEM / .geo file read or write failed     */
+  TDBP_ERR_NOMEM = 5,   /* device or host allocation failed         */
+  TDBP_ERR_LIMIT = 6    /* request exceeds a documented size limit  */
+} tdbp_status;
+
+/* Precision of the per-burst kernel. */
+typedef enum {
+  TDBP_PLAIN_FP64 = 0,  /* FP64 range, phase and accumulator; FP32 lerp    */
+  TDBP_MIXED = 1        /* FP64 per-tile reference, FP32 differential range */
+} tdbp_precision;
+
+/* Storage type of range-compressed samples. */
+typedef enum {
+  TDBP_CF32 = 0,        /* complex float, 8 B/sample                       */
+  TDBP_CF16 = 1,        /* complex IEEE half + FP32 scale per pulse, 4 B   */
+  TDBP_CBF16 = 2        /* complex bfloat16 + FP32 scale per pulse, 4 B    */
+} tdbp_dtype;
+
+/*
+ * Burst geometry: the scalars the reference driver derives per burst
+ * (sentinel/backprojectgpusub.f90:193-235) and the DEM window the entry
+ * point derives from latlons (sentinel/azimuth_compress.cu:295-301, :378-379).
+ */
+typedef struct tdbp_geom {
+  int32_t rawdatalines;   /* L: pulses in the burst                        */
+  int32_t samples;        /* S: samples per pulse (row stride)             */
+  int32_t aperture;       /* A: integration length in pulses               */
+  int32_t numstatevec;    /* orbit state vectors (>= 4)                    */
+  double rngstart, rngend, dmrg, wvl, prf, angc0, angc1, tmid;
+  double xyz_mid[3], vel_mid[3];
+  const double *timeorbit;   /* host [numstatevec]                         */
+  const double *xx, *vv;     /* host [numstatevec][3] (Fortran xx(3,n))    */
+  int32_t demwidth;       /* W: DEM row length                             */
+  int32_t firstline;      /* first DEM line of the window                  */
+  int32_t nlines;         /* window lines                                  */
+  int32_t firstpix, lastpix;  /* pixel range (lastpix may equal W)         */
+  double firstlat, firstlon, deltalat, deltalon;
+  int32_t fit_last_line;  /* 0: like the reference loop, the last window
+                             line gets no aperture fit and stays zero; 1:
+                             fit it like the others                       */
+} tdbp_geom;
+
+/* ---------------------------------------------------------------- drop-in */
+
+/* Replaces sentinel/azimuth_compress.cu:181-214 (called from
+ * sentinel/backprojectgpusub.f90:244-247).  Same 33 by-reference arguments,
+ * same effect: backproject one burst onto DEM lines derived from latlons and
+ * merge into the .geo file behind *fdout (pixel replaced where
+ * |Re| > 1e-18).  Unused, as in the reference: tstart, tend, t, dtaz,
+ * iaperture.  Environment: TDBP_PRECISION=fp64|mixed_fp16|mixed_bf16|mixed_cf32
+ * (default fp64), TDBP_FIT_LAST_LINE=0|1 (default 0).  void like the
+ * reference; failures are reported on stderr and leave the .geo unchanged. */
+void azimuth_compress_(void *burstdata, double *satloc, int *rawdatalines,
+                       int *samplesPerBurst, int *demwidth, int *demlength,
+                       int *fdout, int *fddem, double *deltalat,
+                       double *deltalon, double *firstlat, double *firstlon,
+                       double *latlons, double *timeorbit, double *xx,
+                       double *vv, int *numstatevec, double *rngstart,
+                       double *rngend, double *tstart, double *tend,
+                       double *tmid, double *xyz_mid, double *vel_mid,
+                       double *t, double *dtaz, double *dmrg, double *wvl,
+                       int *aperture, int *iaperture, double *angc0,
+                       double *angc1, double *prf);
+
+/* Status of the last azimuth_compress_ call on this thread (the reference
+ * entry returns nothing). */
+int tdbp_dropin_status(void);
+
+/* ------------------------------------------------------------- runtime */
+
+const char *tdbp_last_error(void);
+/* Number of visible HIP devices (sentinel/howmanygpus.cu:1-9). */
+int tdbp_device_count(void);
+/* Select the device and check it is gfx950; idempotent. */
+int tdbp_init(int device);
+
+void *tdbp_dev_alloc(size_t bytes);
+int tdbp_dev_free(void *d_ptr);
+int tdbp_memcpy_htod(void *d_dst, const void *src, size_t bytes, void *stream);
+int tdbp_memcpy_dtoh(void *dst, const void *d_src, size_t bytes, void *stream);
+int tdbp_memcpy_dtod(void *d_dst, const void *d_src, size_t bytes, void *stream);
+/* Zero device memory with a fill kernel (graph-capture safe). */
+int tdbp_dev_zero(void *d_ptr, size_t bytes, void *stream);
+int tdbp_stream_create(void **stream);
+int tdbp_stream_destroy(void *stream);
+int tdbp_stream_sync(void *stream);
+int tdbp_device_sync(void);
+int tdbp_event_create(void **event);
+int tdbp_event_destroy(void *event);
+int tdbp_event_record(void *event, void *stream);
+int tdbp_event_elapsed_ms(void *start, void *stop, float *ms);
+/* HBM bytes free / total on the current device. */
+int tdbp_mem_info(size_t *free_bytes, size_t *total_bytes);
+
+/* Window derivation of the entry point: sentinel/azimuth_compress.cu:295-301
+ * (lines) and :378-379 (pixels). */
+void tdbp_window(const double *latlons, double firstlat, double firstlon,
+                 double deltalat, double deltalon, int demwidth, int demlength,
+                 int *firstline, int *lastline, int *firstpix, int *lastpix);
+
+/* Host Hermite orbit interpolation at m times (sentinel/intp_orbit.f90:6-20
+ * per time, as sentinel/sentineltimingsub.f90:59-65 builds per-pulse
+ * positions).  pos, vel: [m][3]; either may be NULL. */
+int tdbp_orbit_interp(const double *timeorbit, const double *xx,
+                      const double *vv, int numstatevec, const double *times,
+                      int m, double *pos, double *vel);
+
+/* ---------------------------------------------------- operator API */
+
+/* setxyz (sentinel/azimuth_compress.cu:85-151): ECEF of window pixels
+ * [firstpix, lastpix] (zero elsewhere, like the CPU twin) and the three fit
+ * nodes per line.  d_dem: int16 [nlines][W]; d_xyz: f64 [nlines][W][3];
+ * d_xyzfit: f64 [nlines][9] (may be NULL). */
+int tdbp_setxyz(const tdbp_geom *g, const int16_t *d_dem, double *d_xyz,
+                double *d_xyzfit, void *stream);
+
+/* Aperture-centre parabola per window line (the host loop of
+ * sentinel/azimuth_compress.cu:438-481 with orbitrangetime_), computed on
+ * the device.  d_coef: f64 [nlines][3].  Lines beyond the fit (the last
+ * line when fit_last_line == 0) get coefficients (0, 0, 0), which setazoff
+ * maps to -1. */
+int tdbp_fit_aperture(const tdbp_geom *g, const int16_t *d_dem, double *d_coef,
+                      void *stream);
+
+/* setazoff (sentinel/azimuth_compress.cu:154-178).  d_azoff: f64
+ * [nlines][W], -1 outside [firstpix, lastpix] and outside the valid band. */
+int tdbp_setazoff(const tdbp_geom *g, const double *d_coef, double *d_azoff,
+                  void *stream);
+
+/* Pixel-pulse pairs visited per window line (sum of azend-azstart over
+ * pixels with azoff > 0): the work measure.  d_pairs: int64 [nlines]. */
+int tdbp_pair_count(const tdbp_geom *g, const double *d_coef, int64_t *d_pairs,
+                    void *stream);
+
+/* Range-compressed samples of one burst on the device. */
+typedef struct tdbp_samples {
+  const void *d_data;     /* [L][S] of the dtype                          */
+  const float *d_scale;   /* [L] per-pulse scale (CF16/CBF16), else NULL  */
+  int32_t dtype;          /* tdbp_dtype                                   */
+} tdbp_samples;
+
+/* Convert cf32 samples to CF16/CBF16 with a per-pulse FP32 scale
+ * (scale = max |component| of the pulse / 2^14, 1 for an all-zero pulse). */
+int tdbp_pack_samples(const void *d_cf32, int L, int S, int dtype,
+                      void *d_packed, float *d_scale, void *stream);
+
+/* pixelint on materialised inputs (sentinel/azimuth_compress.cu:33-82):
+ * d_xyz f64 [nlines][W][3], d_azoff f64 [nlines][W], cf32 samples,
+ * d_satloc f64 [L][3]; d_out cf32 [nlines][W] scaled by 1/(azend-azstart).
+ * FP64 range and phase, FP32 lerp. */
+int tdbp_pixelint(const tdbp_geom *g, const void *d_burst,
+                  const double *d_satloc, const double *d_xyz,
+                  const double *d_azoff, void *d_out, void *stream);
+
+/* The fused burst backprojection: DEM -> ECEF -> aperture window -> pulse
+ * sum, one launch after the fit.  d_coef: from tdbp_fit_aperture (NULL: fit
+ * here).  d_out: cf32 [nlines][W]; pixels outside [firstpix, lastpix] or
+ * with no valid window are written 0.  Output scaled by 1/(azend-azstart)
+ * as the reference GPU kernel does. */
+int tdbp_backproject(const tdbp_geom *g, const tdbp_samples *s,
+                     const double *d_satloc, const int16_t *d_dem,
+                     const double *d_coef, void *d_out, int precision,
+                     void *stream);
+
+/* tdbp_backproject with an explicit kernel variant (0 .. count-1). */
+int tdbp_backproject_variant(const tdbp_geom *g, const tdbp_samples *s,
+                             const double *d_satloc, const int16_t *d_dem,
+                             const double *d_coef, void *d_out, int precision,
+                             int variant, void *stream);
+
+/* Autotuner: time every kernel variant on this burst (reps launches each,
+ * after one warm-up), return the fastest in *best and, for TDBP_MIXED, make
+ * it the default of later launches.  ms (optional, [count]) gets the mean
+ * milliseconds per launch of each variant.  Overwrites d_out. */
+int tdbp_autotune(const tdbp_geom *g, const tdbp_samples *s,
+                  const double *d_satloc, const int16_t *d_dem, void *d_out,
+                  int precision, int reps, float *ms, int *best);
+
+/* Kernel variant for TDBP_MIXED launches (autotuner output); -1 = auto. */
+int tdbp_set_variant(int variant);
+int tdbp_variant_count(void);
+/* Describe variant v: rows x cols of the per-wave pixel tile and pulses per
+ * stage. */
+int tdbp_variant_info(int v, int *tile_rows, int *tile_cols, int *stage);
+
+/* ---------------------------------------------------------- batch API */
+
+typedef struct tdbp_batch tdbp_batch;
+
+/* A batch runs every added burst in one persistent launch (plus one fit
+ * launch) fed by a device work queue of (burst, tile) items; the whole
+ * sequence is captured once in a hipGraph and replayed by tdbp_batch_run. */
+tdbp_batch *tdbp_batch_create(int precision, int variant);
+/* Add a burst.  The geometry is copied; device buffers are borrowed and
+ * must stay valid until tdbp_batch_destroy.  Returns the burst index >= 0
+ * or -tdbp_status. */
+int tdbp_batch_add(tdbp_batch *b, const tdbp_geom *g, const tdbp_samples *s,
+                   const double *d_satloc, const int16_t *d_dem, void *d_out);
+/* Run (first call builds the queue and captures the graph). */
+int tdbp_batch_run(tdbp_batch *b, void *stream);
+/* Variant the batch runs (resolved at the first run). */
+int tdbp_batch_variant(const tdbp_batch *b);
+/* Pairs visited by one run (valid after the first run). */
+int64_t tdbp_batch_pairs(const tdbp_batch *b);
+/* Device memory held by the batch itself (queue, descriptors, coef). */
+size_t tdbp_batch_device_bytes(const tdbp_batch *b);
+void tdbp_batch_destroy(tdbp_batch *b);
+
+/* ------------------------------------------- synthetic data (bench/tests) */
+
+/* Band-limited complex Gaussian clutter: white CN(0, sigma^2) from a
+ * counter-based hash of (seed, pulse, sample), convolved along range with
+ * the real FIR taps (host array, ntaps <= 64). */
+int tdbp_synth_clutter(void *d_cf32, int L, int S, uint64_t seed, float sigma,
+                       const float *taps, int ntaps, void *stream);
+
+/* Add point-target echoes: for each target k at ECEF xyz[k] with complex
+ * amplitude (amp[2k], amp[2k+1]), every pulse p gets
+ * a * sinc((r - r_k(p)) / res) * exp(-j 4 pi R_k(p) / wvl) on the samples
+ * r within +-halfwidth of r_k(p) = (R_k(p) - rngstart) / dmrg.  Host
+ * arrays; d_satloc f64 [L][3]. */
+int tdbp_synth_targets(void *d_cf32, const tdbp_geom *g, const double *d_satloc,
+                       const double *xyz, const float *amp, int ntargets,
+                       float res, int halfwidth, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SRG_TDBP_H */

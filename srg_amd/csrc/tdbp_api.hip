@@ -1,0 +1,529 @@
+// tdbp_api.hip -- C ABI of the TDBP engine: runtime helpers, the operator
+// API over device buffers, the batch (persistent launch + device work queue +
+// hipGraph) and the autotuner.  The drop-in azimuth_compress_ lives in
+// tdbp_dropin.hip.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "tdbp_api_internal.h"
+#include "tdbp_geom.h"
+#include "tdbp_internal.h"
+
+namespace tdbp {
+
+static thread_local char g_err[512] = "";
+static int g_variant = -1;
+
+int set_error(int code, const char* fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_err, sizeof g_err, fmt, ap);
+  va_end(ap);
+  return code;
+}
+
+int hip_check(hipError_t e, const char* what) {
+  if (e == hipSuccess) return TDBP_OK;
+  return set_error(TDBP_ERR_HIP, "%s: %s", what, hipGetErrorString(e));
+}
+
+// ------------------------------------------------------------ device init
+
+static std::mutex g_init_mu;
+static std::map<int, int> g_ready;  // device -> status
+
+int ensure_device(int device) {
+  std::lock_guard<std::mutex> lk(g_init_mu);
+  auto it = g_ready.find(device);
+  if (it != g_ready.end()) {
+    if (it->second == TDBP_OK) {
+      hipError_t e = hipSetDevice(device);
+      if (e != hipSuccess) return hip_check(e, "hipSetDevice");
+    }
+    return it->second == TDBP_OK ? TDBP_OK
+                                 : set_error(it->second, "device %d unusable", device);
+  }
+  int n = 0;
+  hipError_t e = hipGetDeviceCount(&n);
+  int st = TDBP_OK;
+  if (e != hipSuccess || n <= device || device < 0) {
+    st = set_error(TDBP_ERR_NODEV, "no HIP device %d (count %d, %s)", device, n,
+                   hipGetErrorString(e));
+  } else if ((e = hipSetDevice(device)) != hipSuccess) {
+    st = hip_check(e, "hipSetDevice");
+  } else {
+    hipDeviceProp_t prop;
+    if ((e = hipGetDeviceProperties(&prop, device)) != hipSuccess) {
+      st = hip_check(e, "hipGetDeviceProperties");
+    } else if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0) {
+      st = set_error(TDBP_ERR_NODEV, "device %d is %s, this build targets gfx950",
+                     device, prop.gcnArchName);
+    }
+  }
+  g_ready[device] = st;
+  return st;
+}
+
+int current_device_ready() {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) dev = 0;
+  return ensure_device(dev);
+}
+
+int cu_count() {
+  int dev = 0, n = 0;
+  (void)hipGetDevice(&dev);
+  if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) !=
+      hipSuccess)
+    return 256;
+  return n > 0 ? n : 256;
+}
+
+// ------------------------------------------------------------ geometry
+
+int validate_geom(const tdbp_geom* g) {
+  if (!g) return set_error(TDBP_ERR_ARG, "null geometry");
+  if (g->rawdatalines < 2 || g->samples < 2)
+    return set_error(TDBP_ERR_ARG, "burst %d x %d too small", g->rawdatalines,
+                     g->samples);
+  if (g->aperture < 0) return set_error(TDBP_ERR_ARG, "aperture %d < 0", g->aperture);
+  if (g->numstatevec < 4 || g->numstatevec > kMaxStateVec)
+    return set_error(TDBP_ERR_ARG, "numstatevec %d outside [4, %d]",
+                     g->numstatevec, kMaxStateVec);
+  if (!g->timeorbit || !g->xx || !g->vv)
+    return set_error(TDBP_ERR_ARG, "orbit arrays missing");
+  if (!(g->timeorbit[1] > g->timeorbit[0]))
+    return set_error(TDBP_ERR_ARG, "orbit times not increasing");
+  if (g->demwidth < 1 || g->nlines < 1)
+    return set_error(TDBP_ERR_ARG, "window %d x %d empty", g->nlines, g->demwidth);
+  if (g->firstpix < 0 || g->lastpix > g->demwidth ||
+      g->lastpix - g->firstpix < 2)
+    return set_error(TDBP_ERR_ARG, "pixel range [%d, %d] invalid for width %d",
+                     g->firstpix, g->lastpix, g->demwidth);
+  if (!(g->dmrg > 0.) || !(g->wvl > 0.) || !(g->prf > 0.))
+    return set_error(TDBP_ERR_ARG, "dmrg/wvl/prf must be positive");
+  // The gate rngstart <= R <= rngend - dmrg must keep sample r+1 inside the
+  // row: (rngend - dmrg - rngstart)/dmrg <= S - 2 (+ rounding).
+  const double rhi = (g->rngend - g->dmrg - g->rngstart) / g->dmrg;
+  if (!(rhi <= g->samples - 2 + 1e-6))
+    return set_error(TDBP_ERR_ARG,
+                     "range window ends at sample %.3f beyond S-2 = %d", rhi,
+                     g->samples - 2);
+  return TDBP_OK;
+}
+
+void fill_burst(const tdbp_geom* g, BurstDev* b) {
+  std::memset(b, 0, sizeof *b);
+  b->L = g->rawdatalines;
+  b->S = g->samples;
+  b->A = g->aperture;
+  b->W = g->demwidth;
+  b->firstline = g->firstline;
+  b->nlines = g->nlines;
+  b->nfit = g->fit_last_line ? g->nlines : g->nlines - 1;
+  b->firstpix = g->firstpix;
+  b->lastpix = g->lastpix;
+  b->nsv = g->numstatevec;
+  b->rngstart = g->rngstart;
+  b->rng_hi = g->rngend - g->dmrg;
+  b->inv_dmrg = 1.0 / g->dmrg;
+  b->two_over_wvl = 2.0 / g->wvl;
+  b->r_hi = std::min((g->rngend - g->dmrg - g->rngstart) / g->dmrg,
+                     (double)(g->samples - 2));
+  b->firstlat = g->firstlat;
+  b->firstlon = g->firstlon;
+  b->deltalat = g->deltalat;
+  b->deltalon = g->deltalon;
+  b->tmid = g->tmid;
+  b->wvl = g->wvl;
+  b->angc0 = g->angc0;
+  b->angc1 = g->angc1;
+  b->prf = g->prf;
+  for (int i = 0; i < 3; ++i) {
+    b->xyz_mid[i] = g->xyz_mid[i];
+    b->vel_mid[i] = g->vel_mid[i];
+  }
+  b->dtype = TDBP_CF32;
+}
+
+void set_tiles(BurstDev* b, int variant) {
+  const Variant& v = kVariants[variant];
+  b->tiles_r = (b->nlines + wg_rows(v) - 1) / wg_rows(v);
+  b->tiles_c = (b->W + wg_cols(v) - 1) / wg_cols(v);
+}
+
+// Orbit arrays packed [t(n) | x(3n) | v(3n)] for the device.
+std::vector<double> pack_orbit(const tdbp_geom* g) {
+  const int n = g->numstatevec;
+  std::vector<double> o(7 * (size_t)n);
+  std::memcpy(o.data(), g->timeorbit, n * sizeof(double));
+  std::memcpy(o.data() + n, g->xx, 3 * n * sizeof(double));
+  std::memcpy(o.data() + 4 * n, g->vv, 3 * n * sizeof(double));
+  return o;
+}
+
+void point_orbit(BurstDev* b, const double* d_orbit, int n) {
+  b->orb_t = d_orbit;
+  b->orb_x = d_orbit + n;
+  b->orb_v = d_orbit + 4 * n;
+}
+
+int check_samples(const tdbp_samples* s, int precision) {
+  if (!s || !s->d_data) return set_error(TDBP_ERR_ARG, "no samples");
+  if (s->dtype < TDBP_CF32 || s->dtype > TDBP_CBF16)
+    return set_error(TDBP_ERR_ARG, "unknown dtype %d", s->dtype);
+  if (precision == TDBP_PLAIN_FP64 && s->dtype != TDBP_CF32)
+    return set_error(TDBP_ERR_ARG, "the FP64 kernel reads cf32 samples only");
+  if (precision != TDBP_PLAIN_FP64 && precision != TDBP_MIXED)
+    return set_error(TDBP_ERR_ARG, "unknown precision %d", precision);
+  if (s->dtype != TDBP_CF32 && !s->d_scale)
+    return set_error(TDBP_ERR_ARG, "packed samples need per-pulse scales");
+  return TDBP_OK;
+}
+
+int default_variant(int precision) {
+  if (precision == TDBP_PLAIN_FP64) {
+    const char* e = std::getenv("TDBP_PLAIN_VARIANT");
+    const int v = e ? std::atoi(e) : 0;
+    return (v >= 0 && v < kNumVariants) ? v : 0;
+  }
+  if (g_variant >= 0) return g_variant;
+  const char* e = std::getenv("TDBP_VARIANT");
+  const int v = e ? std::atoi(e) : 4;
+  return (v >= 0 && v < kNumVariants) ? v : 4;
+}
+
+// Orbit upload + fit on a stream with stream-ordered temporary memory.
+int fit_on_stream(BurstDev& b, const tdbp_geom* g, double* d_coef,
+                  hipStream_t s) {
+  std::vector<double> orb = pack_orbit(g);
+  double* d_orb = nullptr;
+  int st = hip_check(hipMallocAsync((void**)&d_orb, orb.size() * 8, s),
+                     "hipMallocAsync(orbit)");
+  if (st) return st;
+  st = hip_check(hipMemcpyAsync(d_orb, orb.data(), orb.size() * 8,
+                                hipMemcpyHostToDevice, s),
+                 "orbit upload");
+  point_orbit(&b, d_orb, g->numstatevec);
+  if (!st) st = hip_check(launch_fit_one(b, d_coef, s), "fit launch");
+  // The pageable source must outlive the copy.
+  if (!st) st = hip_check(hipStreamSynchronize(s), "fit sync");
+  (void)hipFreeAsync(d_orb, s);
+  b.orb_t = b.orb_x = b.orb_v = nullptr;
+  return st;
+}
+
+int run_backproject(const tdbp_geom* g, const tdbp_samples* smp,
+                    const double* d_satloc, const int16_t* d_dem,
+                    const double* d_coef, void* d_out, int precision,
+                    int variant, hipStream_t s) {
+  int st = validate_geom(g);
+  if (st) return st;
+  if ((st = check_samples(smp, precision))) return st;
+  if (!d_satloc || !d_dem || !d_out)
+    return set_error(TDBP_ERR_ARG, "null device buffer");
+  if ((st = current_device_ready())) return st;
+  BurstDev b;
+  fill_burst(g, &b);
+  b.data = smp->d_data;
+  b.scale = smp->d_scale;
+  b.dtype = smp->dtype;
+  b.satloc = d_satloc;
+  b.dem = d_dem;
+  b.out = (float2*)d_out;
+  if (variant < 0 || variant >= kNumVariants) variant = default_variant(precision);
+  set_tiles(&b, variant);
+  double* own = nullptr;
+  if (!d_coef) {
+    st = hip_check(hipMallocAsync((void**)&own, 24 * (size_t)g->nlines, s),
+                   "hipMallocAsync(coef)");
+    if (st) return st;
+    if ((st = fit_on_stream(b, g, own, s))) {
+      (void)hipFreeAsync(own, s);
+      return st;
+    }
+    d_coef = own;
+  }
+  b.coef = d_coef;
+  hipError_t e = precision == TDBP_PLAIN_FP64 ? launch_plain(b, variant, s)
+                                              : launch_mixed(b, variant, s);
+  st = hip_check(e, "backprojection launch");
+  if (own) (void)hipFreeAsync(own, s);
+  return st;
+}
+
+}  // namespace tdbp
+
+using namespace tdbp;
+
+// ================================================================ C ABI
+
+extern "C" {
+
+const char* tdbp_last_error(void) { return g_err; }
+
+int tdbp_device_count(void) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+  return n;
+}
+
+int tdbp_init(int device) { return ensure_device(device); }
+
+void* tdbp_dev_alloc(size_t bytes) {
+  if (current_device_ready()) return nullptr;
+  void* p = nullptr;
+  if (hipMalloc(&p, bytes ? bytes : 1) != hipSuccess) {
+    set_error(TDBP_ERR_NOMEM, "hipMalloc(%zu) failed", bytes);
+    return nullptr;
+  }
+  return p;
+}
+
+int tdbp_dev_free(void* p) { return hip_check(hipFree(p), "hipFree"); }
+
+int tdbp_memcpy_htod(void* d, const void* h, size_t n, void* s) {
+  int st = hip_check(
+      hipMemcpyAsync(d, h, n, hipMemcpyHostToDevice, (hipStream_t)s), "memcpy H2D");
+  if (st) return st;
+  return hip_check(hipStreamSynchronize((hipStream_t)s), "memcpy H2D sync");
+}
+
+int tdbp_memcpy_dtoh(void* h, const void* d, size_t n, void* s) {
+  int st = hip_check(
+      hipMemcpyAsync(h, d, n, hipMemcpyDeviceToHost, (hipStream_t)s), "memcpy D2H");
+  if (st) return st;
+  return hip_check(hipStreamSynchronize((hipStream_t)s), "memcpy D2H sync");
+}
+
+int tdbp_memcpy_dtod(void* d, const void* src, size_t n, void* s) {
+  return hip_check(
+      hipMemcpyAsync(d, src, n, hipMemcpyDeviceToDevice, (hipStream_t)s),
+      "memcpy D2D");
+}
+
+int tdbp_dev_zero(void* d, size_t n, void* s) {
+  return hip_check(launch_zero(d, n, (hipStream_t)s), "zero");
+}
+
+int tdbp_stream_create(void** s) {
+  if (current_device_ready()) return TDBP_ERR_NODEV;
+  return hip_check(hipStreamCreateWithFlags((hipStream_t*)s, hipStreamNonBlocking),
+                   "hipStreamCreate");
+}
+int tdbp_stream_destroy(void* s) {
+  return hip_check(hipStreamDestroy((hipStream_t)s), "hipStreamDestroy");
+}
+int tdbp_stream_sync(void* s) {
+  return hip_check(hipStreamSynchronize((hipStream_t)s), "hipStreamSynchronize");
+}
+int tdbp_device_sync(void) {
+  return hip_check(hipDeviceSynchronize(), "hipDeviceSynchronize");
+}
+int tdbp_event_create(void** e) {
+  return hip_check(hipEventCreate((hipEvent_t*)e), "hipEventCreate");
+}
+int tdbp_event_destroy(void* e) {
+  return hip_check(hipEventDestroy((hipEvent_t)e), "hipEventDestroy");
+}
+int tdbp_event_record(void* e, void* s) {
+  return hip_check(hipEventRecord((hipEvent_t)e, (hipStream_t)s),
+                   "hipEventRecord");
+}
+int tdbp_event_elapsed_ms(void* a, void* b, float* ms) {
+  return hip_check(hipEventElapsedTime(ms, (hipEvent_t)a, (hipEvent_t)b),
+                   "hipEventElapsedTime");
+}
+int tdbp_mem_info(size_t* f, size_t* t) {
+  return hip_check(hipMemGetInfo(f, t), "hipMemGetInfo");
+}
+
+void tdbp_window(const double* latlons, double firstlat, double firstlon,
+                 double deltalat, double deltalon, int demwidth, int demlength,
+                 int* firstline, int* lastline, int* firstpix, int* lastpix) {
+  int ll = (int)((latlons[0] - firstlat) / deltalat);
+  int fl = (int)((latlons[1] - firstlat) / deltalat);
+  fl = fl < 0 ? 0 : fl;
+  ll = ll < 0 ? 0 : ll;
+  fl = fl > demlength - 1 ? demlength - 1 : fl;
+  ll = ll > demlength - 1 ? demlength - 1 : ll;
+  int fp = (int)((latlons[2] - firstlon) / deltalon);
+  fp = fp < 0 ? 0 : fp;
+  int lp = (int)((latlons[3] - firstlon) / deltalon);
+  lp = lp > demwidth ? demwidth : lp;
+  *firstline = fl;
+  *lastline = ll;
+  *firstpix = fp;
+  *lastpix = lp;
+}
+
+int tdbp_orbit_interp(const double* t, const double* x, const double* v, int n,
+                      const double* times, int m, double* pos, double* vel) {
+  if (!t || !x || !v || !times || n < 4 || m < 0)
+    return set_error(TDBP_ERR_ARG, "orbit_interp: bad arguments");
+  for (int i = 0; i < m; ++i) {
+    double p[3], w[3];
+    orbit_at(t, x, v, n, times[i], p, w);
+    if (pos) std::memcpy(pos + 3 * i, p, sizeof p);
+    if (vel) std::memcpy(vel + 3 * i, w, sizeof w);
+  }
+  return TDBP_OK;
+}
+
+int tdbp_setxyz(const tdbp_geom* g, const int16_t* d_dem, double* d_xyz,
+                double* d_xyzfit, void* stream) {
+  int st = validate_geom(g);
+  if (st) return st;
+  if ((st = current_device_ready())) return st;
+  BurstDev b;
+  fill_burst(g, &b);
+  b.dem = d_dem;
+  return hip_check(launch_setxyz(b, d_xyz, d_xyzfit, (hipStream_t)stream),
+                   "setxyz launch");
+}
+
+int tdbp_fit_aperture(const tdbp_geom* g, const int16_t* d_dem, double* d_coef,
+                      void* stream) {
+  int st = validate_geom(g);
+  if (st) return st;
+  if ((st = current_device_ready())) return st;
+  BurstDev b;
+  fill_burst(g, &b);
+  b.dem = d_dem;
+  return fit_on_stream(b, g, d_coef, (hipStream_t)stream);
+}
+
+int tdbp_setazoff(const tdbp_geom* g, const double* d_coef, double* d_azoff,
+                  void* stream) {
+  int st = validate_geom(g);
+  if (st) return st;
+  if ((st = current_device_ready())) return st;
+  BurstDev b;
+  fill_burst(g, &b);
+  b.coef = d_coef;
+  return hip_check(launch_setazoff(b, d_azoff, (hipStream_t)stream),
+                   "setazoff launch");
+}
+
+int tdbp_pair_count(const tdbp_geom* g, const double* d_coef, int64_t* d_pairs,
+                    void* stream) {
+  int st = validate_geom(g);
+  if (st) return st;
+  if ((st = current_device_ready())) return st;
+  BurstDev b;
+  fill_burst(g, &b);
+  b.coef = d_coef;
+  return hip_check(launch_pair_count(b, d_pairs, (hipStream_t)stream),
+                   "pair count launch");
+}
+
+int tdbp_pack_samples(const void* d_cf32, int L, int S, int dtype,
+                      void* d_packed, float* d_scale, void* stream) {
+  if (!d_cf32 || !d_packed || !d_scale || L < 1 || S < 1 ||
+      (dtype != TDBP_CF16 && dtype != TDBP_CBF16))
+    return set_error(TDBP_ERR_ARG, "pack: bad arguments");
+  int st = current_device_ready();
+  if (st) return st;
+  return hip_check(launch_pack((const float2*)d_cf32, L, S, dtype, d_packed,
+                               d_scale, (hipStream_t)stream),
+                   "pack launch");
+}
+
+int tdbp_pixelint(const tdbp_geom* g, const void* d_burst,
+                  const double* d_satloc, const double* d_xyz,
+                  const double* d_azoff, void* d_out, void* stream) {
+  int st = validate_geom(g);
+  if (st) return st;
+  if ((st = current_device_ready())) return st;
+  if (!d_burst || !d_satloc || !d_xyz || !d_azoff || !d_out)
+    return set_error(TDBP_ERR_ARG, "pixelint: null buffer");
+  BurstDev b;
+  fill_burst(g, &b);
+  b.data = d_burst;
+  b.satloc = d_satloc;
+  b.out = (float2*)d_out;
+  return hip_check(launch_pixelint(b, d_xyz, d_azoff, (hipStream_t)stream),
+                   "pixelint launch");
+}
+
+int tdbp_backproject(const tdbp_geom* g, const tdbp_samples* s,
+                     const double* d_satloc, const int16_t* d_dem,
+                     const double* d_coef, void* d_out, int precision,
+                     void* stream) {
+  return run_backproject(g, s, d_satloc, d_dem, d_coef, d_out, precision, -1,
+                         (hipStream_t)stream);
+}
+
+int tdbp_backproject_variant(const tdbp_geom* g, const tdbp_samples* s,
+                             const double* d_satloc, const int16_t* d_dem,
+                             const double* d_coef, void* d_out, int precision,
+                             int variant, void* stream) {
+  return run_backproject(g, s, d_satloc, d_dem, d_coef, d_out, precision,
+                         variant, (hipStream_t)stream);
+}
+
+int tdbp_set_variant(int v) {
+  if (v < -1 || v >= kNumVariants)
+    return set_error(TDBP_ERR_ARG, "variant %d outside [-1, %d)", v, kNumVariants);
+  g_variant = v;
+  return TDBP_OK;
+}
+int tdbp_variant_count(void) { return kNumVariants; }
+int tdbp_variant_info(int v, int* rows, int* cols, int* stage) {
+  if (v < 0 || v >= kNumVariants) return set_error(TDBP_ERR_ARG, "variant %d", v);
+  if (rows) *rows = wg_rows(kVariants[v]);
+  if (cols) *cols = wg_cols(kVariants[v]);
+  if (stage) *stage = kBlock;
+  return TDBP_OK;
+}
+
+// ------------------------------------------------------------- synthetic
+
+int tdbp_synth_clutter(void* d, int L, int S, uint64_t seed, float sigma,
+                       const float* taps, int ntaps, void* stream) {
+  if (!d || L < 1 || S < 1 || ntaps < 1 || ntaps > 64 || !taps)
+    return set_error(TDBP_ERR_ARG, "synth_clutter: bad arguments");
+  int st = current_device_ready();
+  if (st) return st;
+  return hip_check(launch_synth_clutter((float2*)d, L, S, seed, sigma, taps,
+                                        ntaps, (hipStream_t)stream),
+                   "synth_clutter launch");
+}
+
+int tdbp_synth_targets(void* d, const tdbp_geom* g, const double* d_satloc,
+                       const double* xyz, const float* amp, int ntargets,
+                       float res, int halfwidth, void* stream) {
+  if (!d || !g || !d_satloc || (ntargets > 0 && (!xyz || !amp)) ||
+      halfwidth < 0 || halfwidth > 511 || !(res > 0.f))
+    return set_error(TDBP_ERR_ARG, "synth_targets: bad arguments");
+  int st = current_device_ready();
+  if (st) return st;
+  BurstDev b;
+  std::memset(&b, 0, sizeof b);
+  b.L = g->rawdatalines;
+  b.S = g->samples;
+  b.rngstart = g->rngstart;
+  b.inv_dmrg = 1.0 / g->dmrg;
+  b.two_over_wvl = 2.0 / g->wvl;
+  b.satloc = d_satloc;
+  for (int k = 0; k < ntargets; ++k) {
+    st = hip_check(launch_synth_target((float2*)d, b, xyz[3 * k], xyz[3 * k + 1],
+                                       xyz[3 * k + 2], amp[2 * k], amp[2 * k + 1],
+                                       res, halfwidth, (hipStream_t)stream),
+                   "synth_target launch");
+    if (st) return st;
+  }
+  return TDBP_OK;
+}
+
+}  // extern "C"

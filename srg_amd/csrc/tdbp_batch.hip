@@ -1,0 +1,307 @@
+// tdbp_batch.hip -- many bursts in one persistent launch, and the autotuner.
+//
+// A batch holds one BurstDev per burst in device memory and a work queue of
+// (burst, tile) items split into 8 queues, one per XCD: the (burst, column
+// band) pairs are dealt round-robin to the queues and each band contributes
+// its row tiles in order, so the workgroups of one XCD walk down the same
+// range band and share its samples in their L2.  A workgroup drains its home
+// queue (blockIdx % 8) and then steals from the others.  The launch sequence
+// (zero the queue counters, fit every burst's aperture parabolas, run the
+// persistent kernel) is captured once in a hipGraph and replayed.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdlib>
+#include <cstring>
+#include <vector>
+
+#include "tdbp_api_internal.h"
+#include "tdbp_internal.h"
+
+using namespace tdbp;
+
+struct tdbp_batch {
+  int precision = TDBP_MIXED;
+  int variant = -1;
+  int dtype = -1;
+  std::vector<BurstDev> bursts;
+  std::vector<void*> owned;  // device allocations (orbits, coefs, tables)
+  BurstDev* d_bursts = nullptr;
+  int2* d_items = nullptr;
+  int* d_qbase = nullptr;
+  int* d_counters = nullptr;
+  int nitems = 0, grid = 0, max_lines = 0;
+  int64_t pairs = 0;
+  size_t dev_bytes = 0;
+  hipStream_t cap = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  bool built = false, use_graph = true;
+};
+
+namespace {
+
+int balloc(tdbp_batch* b, void** p, size_t bytes) {
+  if (hipMalloc(p, bytes ? bytes : 1) != hipSuccess)
+    return set_error(TDBP_ERR_NOMEM, "batch hipMalloc(%zu) failed", bytes);
+  b->owned.push_back(*p);
+  b->dev_bytes += bytes;
+  return TDBP_OK;
+}
+
+int enqueue(tdbp_batch* b, hipStream_t s) {
+  int st = hip_check(launch_zero(b->d_counters, 8 * sizeof(int), s), "queue reset");
+  if (st) return st;
+  st = hip_check(launch_batch_fit(b->d_bursts, (int)b->bursts.size(),
+                                  b->max_lines, s),
+                 "batch fit launch");
+  if (st) return st;
+  return hip_check(launch_batch(b->d_bursts, b->d_items, b->d_qbase,
+                                b->d_counters, b->precision, b->variant,
+                                b->dtype, b->grid, s),
+                   "batch launch");
+}
+
+int build(tdbp_batch* b) {
+  int st;
+  if (b->bursts.empty()) return set_error(TDBP_ERR_ARG, "empty batch");
+  if ((st = hip_check(hipStreamCreateWithFlags(&b->cap, hipStreamNonBlocking),
+                      "batch stream")))
+    return st;
+  if (b->variant < 0) b->variant = default_variant(b->precision);
+  b->max_lines = 0;
+  for (auto& bd : b->bursts) {
+    set_tiles(&bd, b->variant);
+    b->max_lines = std::max(b->max_lines, bd.nlines);
+  }
+  const size_t nb = b->bursts.size();
+  if ((st = balloc(b, (void**)&b->d_bursts, nb * sizeof(BurstDev)))) return st;
+  if ((st = hip_check(hipMemcpy(b->d_bursts, b->bursts.data(),
+                                nb * sizeof(BurstDev), hipMemcpyHostToDevice),
+                      "burst table upload")))
+    return st;
+
+  // Fit once eagerly to count the pairs of the work.
+  if ((st = hip_check(launch_batch_fit(b->d_bursts, (int)nb, b->max_lines, b->cap),
+                      "batch fit launch")))
+    return st;
+  size_t tot_lines = 0;
+  for (auto& bd : b->bursts) tot_lines += bd.nlines;
+  int64_t* d_pairs = nullptr;
+  if ((st = balloc(b, (void**)&d_pairs, tot_lines * sizeof(int64_t)))) return st;
+  size_t off = 0;
+  for (auto& bd : b->bursts) {
+    if ((st = hip_check(launch_pair_count(bd, d_pairs + off, b->cap),
+                        "pair count launch")))
+      return st;
+    off += bd.nlines;
+  }
+  std::vector<int64_t> hp(tot_lines);
+  if ((st = hip_check(hipMemcpyAsync(hp.data(), d_pairs, tot_lines * 8,
+                                     hipMemcpyDeviceToHost, b->cap),
+                      "pair download")) ||
+      (st = hip_check(hipStreamSynchronize(b->cap), "pair sync")))
+    return st;
+  b->pairs = 0;
+  for (int64_t v : hp) b->pairs += v;
+
+  // Work queues.
+  std::vector<std::vector<int2>> q(8);
+  int g = 0;
+  for (int bi = 0; bi < (int)nb; ++bi) {
+    const BurstDev& bd = b->bursts[bi];
+    for (int band = 0; band < bd.tiles_c; ++band, ++g) {
+      auto& dst = q[g & 7];
+      for (int rt = 0; rt < bd.tiles_r; ++rt)
+        dst.push_back(make_int2(bi, band * bd.tiles_r + rt));
+    }
+  }
+  std::vector<int2> items;
+  std::vector<int> qbase(9, 0);
+  for (int k = 0; k < 8; ++k) {
+    qbase[k] = (int)items.size();
+    items.insert(items.end(), q[k].begin(), q[k].end());
+  }
+  qbase[8] = (int)items.size();
+  b->nitems = (int)items.size();
+  if ((st = balloc(b, (void**)&b->d_items, items.size() * sizeof(int2))) ||
+      (st = balloc(b, (void**)&b->d_qbase, 9 * sizeof(int))) ||
+      (st = balloc(b, (void**)&b->d_counters, 8 * sizeof(int))))
+    return st;
+  if ((st = hip_check(hipMemcpy(b->d_items, items.data(),
+                                items.size() * sizeof(int2), hipMemcpyHostToDevice),
+                      "item upload")) ||
+      (st = hip_check(hipMemcpy(b->d_qbase, qbase.data(), 9 * sizeof(int),
+                                hipMemcpyHostToDevice),
+                      "queue upload")))
+    return st;
+  b->grid = cu_count() * batch_blocks_per_cu(b->precision, b->variant, b->dtype);
+
+  const char* eg = std::getenv("TDBP_GRAPH");
+  b->use_graph = !(eg && eg[0] == '0');
+  if (b->use_graph) {
+    if ((st = hip_check(hipStreamBeginCapture(b->cap, hipStreamCaptureModeThreadLocal),
+                        "begin capture")))
+      return st;
+    const int est = enqueue(b, b->cap);
+    hipGraph_t gr = nullptr;
+    st = hip_check(hipStreamEndCapture(b->cap, &gr), "end capture");
+    if (est) return est;
+    if (st) return st;
+    b->graph = gr;
+    if ((st = hip_check(hipGraphInstantiate(&b->exec, gr, nullptr, nullptr, 0),
+                        "graph instantiate")))
+      return st;
+  }
+  b->built = true;
+  return TDBP_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+tdbp_batch* tdbp_batch_create(int precision, int variant) {
+  if (precision != TDBP_PLAIN_FP64 && precision != TDBP_MIXED) {
+    set_error(TDBP_ERR_ARG, "unknown precision %d", precision);
+    return nullptr;
+  }
+  if (variant < -1 || variant >= kNumVariants) {
+    set_error(TDBP_ERR_ARG, "variant %d", variant);
+    return nullptr;
+  }
+  if (current_device_ready()) return nullptr;
+  tdbp_batch* b = new tdbp_batch;
+  b->precision = precision;
+  b->variant = variant;
+  return b;
+}
+
+int tdbp_batch_add(tdbp_batch* b, const tdbp_geom* g, const tdbp_samples* s,
+                   const double* d_satloc, const int16_t* d_dem, void* d_out) {
+  if (!b) return -set_error(TDBP_ERR_ARG, "null batch");
+  if (b->built) return -set_error(TDBP_ERR_ARG, "batch already built");
+  int st = validate_geom(g);
+  if (st) return -st;
+  if ((st = check_samples(s, b->precision))) return -st;
+  if (!d_satloc || !d_dem || !d_out)
+    return -set_error(TDBP_ERR_ARG, "null device buffer");
+  if (b->dtype >= 0 && s->dtype != b->dtype)
+    return -set_error(TDBP_ERR_ARG, "a batch holds one sample dtype");
+  b->dtype = s->dtype;
+  BurstDev bd;
+  fill_burst(g, &bd);
+  bd.data = s->d_data;
+  bd.scale = s->d_scale;
+  bd.dtype = s->dtype;
+  bd.satloc = d_satloc;
+  bd.dem = d_dem;
+  bd.out = (float2*)d_out;
+  std::vector<double> orb = pack_orbit(g);
+  double* d_orb = nullptr;
+  double* d_coef = nullptr;
+  if ((st = balloc(b, (void**)&d_orb, orb.size() * 8)) ||
+      (st = balloc(b, (void**)&d_coef, 24 * (size_t)g->nlines)))
+    return -st;
+  if ((st = hip_check(hipMemcpy(d_orb, orb.data(), orb.size() * 8,
+                                hipMemcpyHostToDevice),
+                      "orbit upload")))
+    return -st;
+  point_orbit(&bd, d_orb, g->numstatevec);
+  bd.coef = d_coef;
+  b->bursts.push_back(bd);
+  return (int)b->bursts.size() - 1;
+}
+
+int tdbp_batch_run(tdbp_batch* b, void* stream) {
+  if (!b) return set_error(TDBP_ERR_ARG, "null batch");
+  int st = current_device_ready();
+  if (st) return st;
+  if (!b->built && (st = build(b))) return st;
+  if (b->use_graph)
+    return hip_check(hipGraphLaunch(b->exec, (hipStream_t)stream), "graph launch");
+  return enqueue(b, (hipStream_t)stream);
+}
+
+int64_t tdbp_batch_pairs(const tdbp_batch* b) { return b ? b->pairs : 0; }
+
+size_t tdbp_batch_device_bytes(const tdbp_batch* b) {
+  return b ? b->dev_bytes : 0;
+}
+
+int tdbp_batch_variant(const tdbp_batch* b) { return b ? b->variant : -1; }
+
+void tdbp_batch_destroy(tdbp_batch* b) {
+  if (!b) return;
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  if (b->cap) {
+    (void)hipStreamSynchronize(b->cap);
+    (void)hipStreamDestroy(b->cap);
+  }
+  (void)hipDeviceSynchronize();
+  for (void* p : b->owned) (void)hipFree(p);
+  delete b;
+}
+
+// Time every variant on one burst and keep the fastest for later TDBP_MIXED
+// launches.  ms (optional) gets the mean time of each variant.
+int tdbp_autotune(const tdbp_geom* g, const tdbp_samples* s,
+                  const double* d_satloc, const int16_t* d_dem, void* d_out,
+                  int precision, int reps, float* ms, int* best) {
+  int st = validate_geom(g);
+  if (st) return st;
+  if ((st = check_samples(s, precision))) return st;
+  if ((st = current_device_ready())) return st;
+  reps = reps < 1 ? 1 : reps;
+  hipStream_t str = nullptr;
+  if ((st = hip_check(hipStreamCreateWithFlags(&str, hipStreamNonBlocking),
+                      "tune stream")))
+    return st;
+  double* d_coef = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  int bestv = -1;
+  float bestms = 3.4e38f;
+  do {
+    if ((st = hip_check(hipMalloc((void**)&d_coef, 24 * (size_t)g->nlines),
+                        "tune coef")))
+      break;
+    BurstDev b;
+    fill_burst(g, &b);
+    b.dem = d_dem;
+    if ((st = fit_on_stream(b, g, d_coef, str))) break;
+    if ((st = hip_check(hipEventCreate(&e0), "event")) ||
+        (st = hip_check(hipEventCreate(&e1), "event")))
+      break;
+    for (int v = 0; v < kNumVariants && !st; ++v) {
+      st = run_backproject(g, s, d_satloc, d_dem, d_coef, d_out, precision, v,
+                           str);  // warm-up
+      if (st) break;
+      (void)hipEventRecord(e0, str);
+      for (int r = 0; r < reps && !st; ++r)
+        st = run_backproject(g, s, d_satloc, d_dem, d_coef, d_out, precision, v,
+                             str);
+      (void)hipEventRecord(e1, str);
+      if (!st) st = hip_check(hipEventSynchronize(e1), "tune sync");
+      float t = 0.f;
+      if (!st) (void)hipEventElapsedTime(&t, e0, e1);
+      t /= reps;
+      if (ms) ms[v] = t;
+      if (!st && t < bestms) {
+        bestms = t;
+        bestv = v;
+      }
+    }
+  } while (0);
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  (void)hipStreamSynchronize(str);
+  if (d_coef) (void)hipFree(d_coef);
+  (void)hipStreamDestroy(str);
+  if (st) return st;
+  if (best) *best = bestv;
+  if (precision == TDBP_MIXED) tdbp_set_variant(bestv);
+  return TDBP_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,84 @@
+// tdbp_internal.h -- device-side burst descriptor and launcher interface
+// shared by the kernel and API translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#include "../../include/tdbp.h"
+
+namespace tdbp {
+
+constexpr int kWave = 64;        // CDNA wavefront
+constexpr int kBlock = 256;      // 4 waves per workgroup
+constexpr int kMaxStateVec = 1000;  // reference arrays timeorbit(1000)
+
+// Everything one burst needs on the device.  Built on the host from a
+// tdbp_geom; lives in kernel arguments (per-burst launch) or in a device
+// array indexed by the work queue (batch).
+struct BurstDev {
+  const void* data;        // samples [L][S] of `dtype`
+  const float* scale;      // per-pulse scale (CF16/CBF16) or nullptr
+  const double* satloc;    // [L][3]
+  const int16_t* dem;      // [nlines][W]
+  const double* coef;      // [nlines][3]
+  float2* out;             // [nlines][W]
+  const double* orb_t;     // orbit arrays on device ([n], [n][3], [n][3])
+  const double* orb_x;
+  const double* orb_v;
+  int L, S, A, W;
+  int firstline, nlines, nfit, firstpix, lastpix;
+  int dtype, nsv;
+  int tiles_r, tiles_c;    // tile grid of this burst for the chosen variant
+  double rngstart, rng_hi, inv_dmrg, two_over_wvl, r_hi;
+  double firstlat, firstlon, deltalat, deltalon;
+  double tmid, wvl, angc0, angc1, prf;
+  double xyz_mid[3], vel_mid[3];
+};
+
+// Mixed-kernel variants: per-wave pixel tile rows (cols = 64 / rows) and
+// how the 4 waves of a workgroup are stacked.
+struct Variant {
+  int tile_rows;     // rows of a wave tile
+  int stack_rows;    // 1: waves stacked along rows, 0: along columns
+};
+constexpr int kNumVariants = 8;
+extern const Variant kVariants[kNumVariants];
+
+inline int wg_rows(const Variant& v) {
+  return v.stack_rows ? 4 * v.tile_rows : v.tile_rows;
+}
+inline int wg_cols(const Variant& v) {
+  return v.stack_rows ? kWave / v.tile_rows : 4 * (kWave / v.tile_rows);
+}
+
+// Launchers (tdbp_kernels.hip).  All return hipError_t of the launch.
+hipError_t launch_fit_one(const BurstDev& b, double* coef, hipStream_t s);
+hipError_t launch_setxyz(const BurstDev& b, double* xyz, double* xyzfit,
+                         hipStream_t s);
+hipError_t launch_setazoff(const BurstDev& b, double* azoff, hipStream_t s);
+hipError_t launch_pair_count(const BurstDev& b, int64_t* pairs, hipStream_t s);
+hipError_t launch_pixelint(const BurstDev& b, const double* xyz,
+                           const double* azoff, hipStream_t s);
+hipError_t launch_plain(const BurstDev& b, int variant, hipStream_t s);
+hipError_t launch_mixed(const BurstDev& b, int variant, hipStream_t s);
+hipError_t launch_batch(const BurstDev* d_bursts, const int2* d_items,
+                        const int* d_qbase, int* d_counters, int precision,
+                        int variant, int dtype, int grid, hipStream_t s);
+hipError_t launch_batch_fit(const BurstDev* d_bursts, int nbursts,
+                            int max_lines, hipStream_t s);
+hipError_t launch_zero(void* p, size_t bytes, hipStream_t s);
+hipError_t launch_pack(const float2* in, int L, int S, int dtype, void* out,
+                       float* scale, hipStream_t s);
+hipError_t launch_synth_clutter(float2* d, int L, int S, uint64_t seed,
+                                float sigma, const float* taps, int ntaps,
+                                hipStream_t s);
+hipError_t launch_synth_target(float2* d, const BurstDev& b, double tx,
+                               double ty, double tz, float ar, float ai,
+                               float res, int halfwidth, hipStream_t s);
+
+// Blocks per CU the batch kernel keeps resident.
+int batch_blocks_per_cu(int precision, int variant, int dtype);
+
+}  // namespace tdbp

@@ -1,0 +1,854 @@
+// tdbp_kernels.hip -- gfx950 kernels of the TDBP engine.
+//
+// Hot path: one workgroup (4 waves of 64) owns a tile of output pixels of one
+// burst.  Each lane owns one pixel: it computes the pixel's ECEF position from
+// the int16 DEM and its pulse window from the line parabola (the reference's
+// setxyz and setazoff, fused), then sums over its pulses.  Pulses are looped
+// wave-uniformly, so per-pulse data are wave-uniform operands (scalar loads
+// in the FP64 kernel, LDS broadcast reads in the mixed kernel) and only the
+// range-sample gather is a per-lane memory access.
+//
+//  * k_plain : FP64 range, cycle-reduced FP64 phase, FP32 lerp, FP64
+//              accumulator -- the drop-in and the judge.
+//  * k_mixed : per workgroup, a FP64 reference point x0; per pulse (FP64,
+//              once per workgroup) range R0, unit look vector u, and the
+//              phase/range bases; per pixel-pulse only FP32:
+//                 a  = u.d (d = x - x0),  R = R0 + a + (|d|^2 - a^2)/(2(R0+a))
+//              with the linear phase split as K u_ref.d (per pixel, FP64
+//              once) + K (u - u_ref).d (small, FP32), hardware sin/cos on
+//              revolutions, stage-blocked FP32 sums joined by Kahan addition.
+//
+// Tile order is XCD-aware: blocks are dealt to the 8 XCDs round-robin, so
+// block b is remapped to tile index x*q + min(x, r) + j (x = b % 8) and every
+// XCD walks a contiguous run of column-band-major tiles; tiles in one column
+// band share range samples, which then stay in that XCD's L2.
+#include <hip/hip_fp16.h>
+#include <hip/hip_runtime.h>
+
+#include "tdbp_geom.h"
+#include "tdbp_internal.h"
+
+namespace tdbp {
+
+const Variant kVariants[kNumVariants] = {
+    {8, 0}, {4, 0}, {2, 0}, {1, 0}, {8, 1}, {4, 1}, {2, 1}, {1, 1}};
+
+typedef const double __attribute__((address_space(4)))* const_dptr;
+
+// ------------------------------------------------------------------ helpers
+
+__device__ __forceinline__ int xcd_tile(int b, int nb) {
+  const int q = nb >> 3, r = nb & 7, x = b & 7, j = b >> 3;
+  return x * q + (x < r ? x : r) + j;
+}
+
+__device__ __forceinline__ int wave_min(int v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o, 64));
+  return __builtin_amdgcn_readfirstlane(v);
+}
+__device__ __forceinline__ int wave_max(int v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, 64));
+  return __builtin_amdgcn_readfirstlane(v);
+}
+
+// Pixel owned by this thread in tile `tile` of burst b, with its window.
+struct PixelCtx {
+  int row, col;      // window line, DEM column
+  bool inwin;        // inside the [nlines][W] output array
+  bool computed;     // azoff > 0: the reference computes this pixel
+  int azs, aze;      // pulse window, empty if not computed
+};
+
+template <int TR, bool STACK>
+__device__ __forceinline__ PixelCtx pixel_of(const BurstDev& b, int tile) {
+  constexpr int TC = kWave / TR;
+  constexpr int WR = STACK ? 4 * TR : TR;
+  constexpr int WC = STACK ? TC : 4 * TC;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int band = tile / b.tiles_r;
+  const int rt = tile - band * b.tiles_r;
+  PixelCtx p;
+  p.row = rt * WR + (STACK ? wave * TR : 0) + lane / TC;
+  p.col = band * WC + (STACK ? 0 : wave * TC) + lane % TC;
+  p.inwin = p.row < b.nlines && p.col < b.W;
+  p.azs = 0;
+  p.aze = 0;
+  p.computed = false;
+  if (p.inwin && p.row < b.nfit && p.col >= b.firstpix && p.col <= b.lastpix) {
+    const double* c = b.coef + 3 * p.row;
+    const double az = pixel_azoff(c, p.col, b.firstpix, b.lastpix, b.A, b.L);
+    if (az > 0.) {
+      p.computed = true;
+      pulse_window(az, b.A, b.L, &p.azs, &p.aze);
+    }
+  }
+  return p;
+}
+
+__device__ __forceinline__ void pixel_ecef(const BurstDev& b, int row, int col,
+                                           double* x) {
+  const double h = b.dem[(size_t)row * b.W + col];
+  dem_to_ecef(row, b.firstline, col, b.firstlat, b.deltalat, b.firstlon,
+              b.deltalon, h, x);
+}
+
+// Reference GPU output: cf32(acc) * float(1/(azend-azstart))
+// (sentinel/azimuth_compress.cu:78-79).
+__device__ __forceinline__ void store_out(const BurstDev& b, const PixelCtx& p,
+                                          float re, float im) {
+  if (!p.inwin) return;
+  float2 o = make_float2(0.f, 0.f);
+  if (p.computed) {
+    const float s = (float)(1. / (p.aze - p.azs));
+    o = make_float2(re * s, im * s);
+  }
+  b.out[(size_t)p.row * b.W + p.col] = o;
+}
+
+// ------------------------------------------------------- FP64 pair (judge)
+
+// One pixel-pulse pair in FP64: range, gate, FP32 lerp of the two samples,
+// phase exp(+j 4 pi R / lambda) reduced in cycles before sincospi.
+__device__ __forceinline__ void plain_pair(const float2* __restrict__ row,
+                                           const double* x, double sx,
+                                           double sy, double sz,
+                                           const BurstDev& b, double& ar,
+                                           double& ai) {
+  const double dx = x[0] - sx, dy = x[1] - sy, dz = x[2] - sz;
+  const double R = sqrt(dx * dx + dy * dy + dz * dz);
+  if (R >= b.rngstart && R <= b.rng_hi) {
+    const double r = (R - b.rngstart) * b.inv_dmrg;
+    const double fl = floor(r);
+    int i = (int)fl;
+    i = i > b.S - 2 ? b.S - 2 : i;
+    const double fr = r - fl;
+    const float2 b0 = row[i], b1 = row[i + 1];
+    const float w0 = (float)(1.0 - fr), w1 = (float)fr;
+    const float cr = b0.x * w0 + b1.x * w1;
+    const float ci = b0.y * w0 + b1.y * w1;
+    const double cyc = R * b.two_over_wvl;
+    const double f = cyc - rint(cyc);
+    double sn, cs;
+    sincospi(2.0 * f, &sn, &cs);
+    ar += (double)cr * cs - (double)ci * sn;
+    ai += (double)cr * sn + (double)ci * cs;
+  }
+}
+
+template <int TR, bool STACK>
+__device__ __forceinline__ void plain_tile(const BurstDev& b, int tile) {
+  PixelCtx p = pixel_of<TR, STACK>(b, tile);
+  double x[3] = {0., 0., 0.};
+  const bool live = p.aze > p.azs;
+  if (live) pixel_ecef(b, p.row, p.col, x);
+  const int lo = wave_min(live ? p.azs : 0x7fffffff);
+  const int hi = wave_max(live ? p.aze : -1);
+  double ar = 0., ai = 0.;
+  const float2* base = (const float2*)b.data;
+  const const_dptr sat = (const_dptr)b.satloc;
+  const unsigned n = (unsigned)(p.aze - p.azs);
+  for (int q = lo; q < hi; ++q) {
+    const double sx = sat[3 * q], sy = sat[3 * q + 1], sz = sat[3 * q + 2];
+    if ((unsigned)(q - p.azs) < n)
+      plain_pair(base + (size_t)q * b.S, x, sx, sy, sz, b, ar, ai);
+  }
+  store_out(b, p, (float)ar, (float)ai);
+}
+
+// ------------------------------------------------------- mixed precision
+
+// Sample pair loads, returned as FP32 (re0, im0, re1, im1) times nothing:
+// the per-pulse scale is applied to the phasor.
+template <int DT>
+__device__ __forceinline__ float4 load_pair(const void* rowp, int i) {
+  if constexpr (DT == TDBP_CF32) {
+    const float2* r = (const float2*)rowp;
+    const float2 a = r[i], c = r[i + 1];
+    return make_float4(a.x, a.y, c.x, c.y);
+  } else if constexpr (DT == TDBP_CF16) {
+    const uint32_t* r = (const uint32_t*)rowp;
+    uint2 w;
+    __builtin_memcpy(&w, r + i, 8);
+    typedef _Float16 h2 __attribute__((ext_vector_type(2)));
+    h2 u0, u1;
+    __builtin_memcpy(&u0, &w.x, 4);
+    __builtin_memcpy(&u1, &w.y, 4);
+    return make_float4((float)u0.x, (float)u0.y, (float)u1.x, (float)u1.y);
+  } else {
+    const uint32_t* r = (const uint32_t*)rowp;
+    uint2 w;
+    __builtin_memcpy(&w, r + i, 8);
+    return make_float4(__uint_as_float(w.x << 16),
+                       __uint_as_float(w.x & 0xffff0000u),
+                       __uint_as_float(w.y << 16),
+                       __uint_as_float(w.y & 0xffff0000u));
+  }
+}
+
+template <int DT>
+__device__ __forceinline__ int elem_bytes() {
+  return DT == TDBP_CF32 ? 8 : 4;
+}
+
+// Per-pulse record in LDS (3 x float4, written once per workgroup stage):
+//   r0 = (du.x, du.y, du.z, c0)      du = u_p - u_ref, c0 = frac(K R0) cycles
+//   r1 = (f0, i0 bits, 1/R0, 1/(2 R0))   (R0 - rngstart)/dmrg = i0 + f0
+//   r2 = (lim, scale, -, -)          gate: rho <= lim  <=>  r <= r_hi
+struct PulseRec {
+  float4 r0, r1, r2;
+};
+
+constexpr int kStage = kBlock;  // pulses per stage = one per thread
+
+// Workgroup scratch of the mixed kernel besides the pulse records.
+struct TileShared {
+  double x0[3];      // reference point
+  int key, wlo, whi;
+};
+
+template <int TR, bool STACK, int DT>
+__device__ __forceinline__ void mixed_tile(const BurstDev& b, int tile,
+                                           PulseRec* rec, TileShared* sh) {
+  PixelCtx p = pixel_of<TR, STACK>(b, tile);
+  const int tid = threadIdx.x;
+  const bool live = p.aze > p.azs;
+  double x[3] = {0., 0., 0.};
+  if (live) pixel_ecef(b, p.row, p.col, x);
+
+  // Workgroup window and reference pixel (the live pixel nearest the tile
+  // centre, ties to the lowest thread).
+  constexpr int TC = kWave / TR;
+  const int lane = tid & 63, wave = tid >> 6;
+  const int wr = (STACK ? wave * TR : 0) + lane / TC;
+  const int wc = (STACK ? 0 : wave * TC) + lane % TC;
+  constexpr int WR = STACK ? 4 * TR : TR, WC = STACK ? TC : 4 * TC;
+  const int key = (abs(2 * wr - WR) + abs(2 * wc - WC)) * 256 + tid;
+  if (tid == 0) {
+    sh->key = 0x7fffffff;
+    sh->wlo = 0x7fffffff;
+    sh->whi = -1;
+  }
+  __syncthreads();
+  if (live) {
+    atomicMin(&sh->key, key);
+    atomicMin(&sh->wlo, p.azs);
+    atomicMax(&sh->whi, p.aze);
+  }
+  __syncthreads();
+  const int ref = sh->key;
+  const int wlo = sh->wlo, whi = sh->whi;
+  if (ref == 0x7fffffff) {  // nothing to compute in this tile
+    store_out(b, p, 0.f, 0.f);
+    return;
+  }
+  if ((ref & 255) == tid) {
+    sh->x0[0] = x[0];
+    sh->x0[1] = x[1];
+    sh->x0[2] = x[2];
+  }
+  __syncthreads();
+  const double x0[3] = {sh->x0[0], sh->x0[1], sh->x0[2]};
+
+  // Unit look vector at the window's central pulse.
+  const int pc = (wlo + whi - 1) >> 1;
+  double uref[3];
+  {
+    const double* s = b.satloc + 3 * pc;
+    double u[3] = {x0[0] - s[0], x0[1] - s[1], x0[2] - s[2]};
+    const double inv = 1.0 / sqrt(u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
+    uref[0] = u[0] * inv;
+    uref[1] = u[1] * inv;
+    uref[2] = u[2] * inv;
+  }
+
+  // Per-pixel constants.
+  const double d64[3] = {x[0] - x0[0], x[1] - x0[1], x[2] - x0[2]};
+  const float dx = (float)d64[0], dy = (float)d64[1], dz = (float)d64[2];
+  const float bb = (float)(d64[0] * d64[0] + d64[1] * d64[1] + d64[2] * d64[2]);
+  const double aref64 = uref[0] * d64[0] + uref[1] * d64[1] + uref[2] * d64[2];
+  const float aref = (float)aref64;
+  const double cp64 = aref64 * b.two_over_wvl;
+  const float cpix = (float)(cp64 - rint(cp64));
+  const float Kf = (float)b.two_over_wvl;
+  const float invd = (float)b.inv_dmrg;
+
+  const int qlo = wave_min(live ? p.azs : 0x7fffffff);
+  const int qhi = wave_max(live ? p.aze : -1);
+  const unsigned n = (unsigned)(p.aze - p.azs);
+  const char* base = (const char*)b.data;
+  const size_t rowbytes = (size_t)b.S * elem_bytes<DT>();
+  const unsigned smax = (unsigned)(b.S - 2);
+
+  float acc_r = 0.f, acc_i = 0.f, cmp_r = 0.f, cmp_i = 0.f;
+  for (int p0 = wlo; p0 < whi; p0 += kStage) {
+    // Stage prologue (FP64): one pulse per thread.
+    const int q = p0 + tid;
+    if (q < whi) {
+      const double* s = b.satloc + 3 * q;
+      const double u0 = x0[0] - s[0], u1 = x0[1] - s[1], u2 = x0[2] - s[2];
+      const double R0 = sqrt(u0 * u0 + u1 * u1 + u2 * u2);
+      const double inv = 1.0 / R0;
+      const double c = R0 * b.two_over_wvl;
+      const double rb = (R0 - b.rngstart) * b.inv_dmrg;
+      const double i0 = floor(rb);
+      PulseRec r;
+      r.r0 = make_float4((float)(u0 * inv - uref[0]), (float)(u1 * inv - uref[1]),
+                         (float)(u2 * inv - uref[2]), (float)(c - rint(c)));
+      r.r1 = make_float4((float)(rb - i0), __int_as_float((int)i0), (float)inv,
+                         (float)(0.5 * inv));
+      const float sc = (DT == TDBP_CF32 || b.scale == nullptr) ? 1.f : b.scale[q];
+      r.r2 = make_float4((float)(b.r_hi - i0), sc, 0.f, 0.f);
+      rec[tid] = r;
+    }
+    __syncthreads();
+    const int qa = max(p0, qlo), qb = min(p0 + kStage, qhi);
+    float sr = 0.f, si = 0.f;
+    for (int qq = qa; qq < qb; ++qq) {
+      const PulseRec r = rec[qq - p0];
+      const float da = fmaf(r.r0.x, dx, fmaf(r.r0.y, dy, r.r0.z * dz));
+      const float a = aref + da;
+      const float t = fmaf(-a, a, bb);
+      const float w = fmaf(-a, r.r1.z, 1.0f);
+      const float del = t * r.r1.w * w;
+      const float rho = fmaf(a + del, invd, r.r1.x);
+      const float fl = floorf(rho);
+      const int idx = (int)fl + __float_as_int(r.r1.y);
+      if ((unsigned)(qq - p.azs) < n && (unsigned)idx <= smax &&
+          rho <= r.r2.x) {
+        const float fr = rho - fl;
+        const float4 v = load_pair<DT>(base + (size_t)qq * rowbytes, idx);
+        const float cr = fmaf(fr, v.z - v.x, v.x);
+        const float ci = fmaf(fr, v.w - v.y, v.y);
+        const float cyc = fmaf(Kf, da + del, r.r0.w + cpix);
+        const float ph = __builtin_amdgcn_fractf(cyc);
+        float sn = __builtin_amdgcn_sinf(ph), cs = __builtin_amdgcn_cosf(ph);
+        if constexpr (DT != TDBP_CF32) {
+          sn *= r.r2.y;
+          cs *= r.r2.y;
+        }
+        sr = fmaf(cr, cs, sr);
+        sr = fmaf(-ci, sn, sr);
+        si = fmaf(cr, sn, si);
+        si = fmaf(ci, cs, si);
+      }
+    }
+    // Kahan-compensated join of the stage sums.
+    {
+      const float yr = sr - cmp_r, tr = acc_r + yr;
+      cmp_r = (tr - acc_r) - yr;
+      acc_r = tr;
+      const float yi = si - cmp_i, ti = acc_i + yi;
+      cmp_i = (ti - acc_i) - yi;
+      acc_i = ti;
+    }
+    __syncthreads();
+  }
+  store_out(b, p, acc_r - cmp_r, acc_i - cmp_i);
+}
+
+// ----------------------------------------------------------- kernels
+
+template <int TR, bool STACK>
+__global__ __launch_bounds__(kBlock) void k_plain(BurstDev b) {
+  const int nb = b.tiles_r * b.tiles_c;
+  plain_tile<TR, STACK>(b, xcd_tile(blockIdx.x, nb));
+}
+
+template <int TR, bool STACK, int DT>
+__global__ __launch_bounds__(kBlock) void k_mixed(BurstDev b) {
+  __shared__ PulseRec rec[kStage];
+  __shared__ TileShared sh;
+  const int nb = b.tiles_r * b.tiles_c;
+  mixed_tile<TR, STACK, DT>(b, xcd_tile(blockIdx.x, nb), rec, &sh);
+}
+
+// Persistent batch kernel: workgroup on XCD x = blockIdx % 8 drains queue x,
+// then steals from the others.  Queue k holds items [qbase[k], qbase[k+1]).
+template <int PREC, int TR, bool STACK, int DT>
+__global__ __launch_bounds__(kBlock) void k_batch(const BurstDev* __restrict__ bursts,
+                                                  const int2* __restrict__ items,
+                                                  const int* __restrict__ qbase,
+                                                  int* counters) {
+  __shared__ PulseRec rec[PREC == TDBP_MIXED ? kStage : 1];
+  __shared__ TileShared sh;
+  __shared__ int item_s;
+  const int home = blockIdx.x & 7;
+  int qi = 0;
+  while (qi < 8) {
+    const int qk = (home + qi) & 7;
+    if (threadIdx.x == 0) {
+      const int k = atomicAdd(&counters[qk], 1);
+      item_s = (k < qbase[qk + 1] - qbase[qk]) ? qbase[qk] + k : -1;
+    }
+    __syncthreads();
+    const int it = item_s;
+    __syncthreads();
+    if (it < 0) {
+      ++qi;
+      continue;
+    }
+    const int2 w = items[it];
+    const BurstDev& bd = bursts[w.x];
+    if constexpr (PREC == TDBP_MIXED) {
+      mixed_tile<TR, STACK, DT>(bd, w.y, rec, &sh);
+    } else {
+      plain_tile<TR, STACK>(bd, w.y);
+    }
+    __syncthreads();
+  }
+}
+
+// One thread per window line: three zero-Doppler solves, the aperture
+// centres, the parabola.  Lines at or beyond nfit get (0, 0, 0).
+__device__ __forceinline__ void fit_line(const BurstDev& b, int line,
+                                         double* coef) {
+  double* c = coef + 3 * line;
+  if (line >= b.nfit) {
+    c[0] = 0.;
+    c[1] = 0.;
+    c[2] = 0.;
+    return;
+  }
+  FitParams f;
+  f.tmid = b.tmid;
+  f.wvl = b.wvl;
+  f.angc0 = b.angc0;
+  f.angc1 = b.angc1;
+  f.prf = b.prf;
+  f.rawdatalines = b.L;
+  for (int i = 0; i < 3; ++i) {
+    f.xyz_mid[i] = b.xyz_mid[i];
+    f.vel_mid[i] = b.vel_mid[i];
+  }
+  const int step = fit_node_step(b.firstpix, b.lastpix);
+  int y[3];
+  for (int node = 0; node < 3; ++node) {
+    const int pix = b.firstpix + node * step;
+    double xyz[3];
+    pixel_ecef(b, line, pix, xyz);
+    y[node] = aperture_centre(xyz, b.orb_t, b.orb_x, b.orb_v, b.nsv, f);
+  }
+  parabola(y[0], y[1], y[2], c);
+}
+
+__global__ void k_fit(BurstDev b, double* coef) {
+  const int line = blockIdx.x * blockDim.x + threadIdx.x;
+  if (line < b.nlines) fit_line(b, line, coef);
+}
+
+__global__ void k_batch_fit(const BurstDev* __restrict__ bursts) {
+  const BurstDev& b = bursts[blockIdx.y];
+  const int line = blockIdx.x * blockDim.x + threadIdx.x;
+  if (line < b.nlines) fit_line(b, line, const_cast<double*>(b.coef));
+}
+
+// setxyz: ECEF of [firstpix, lastpix], zero elsewhere; fit nodes per line.
+__global__ void k_setxyz(BurstDev b, double* xyz, double* xyzfit) {
+  const size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t total = (size_t)b.nlines * b.W;
+  if (k >= total) return;
+  const int row = (int)(k / b.W), col = (int)(k - (size_t)row * b.W);
+  double o[3] = {0., 0., 0.};
+  if (col >= b.firstpix && col <= b.lastpix) pixel_ecef(b, row, col, o);
+  xyz[3 * k] = o[0];
+  xyz[3 * k + 1] = o[1];
+  xyz[3 * k + 2] = o[2];
+  if (xyzfit && col >= b.firstpix && col <= b.lastpix) {
+    const int step = fit_node_step(b.firstpix, b.lastpix);
+    for (int node = 0; node < 3; ++node)
+      if (col == b.firstpix + node * step)
+        for (int i = 0; i < 3; ++i) xyzfit[9 * row + 3 * node + i] = o[i];
+  }
+}
+
+__global__ void k_setazoff(BurstDev b, double* azoff) {
+  const size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t total = (size_t)b.nlines * b.W;
+  if (k >= total) return;
+  const int row = (int)(k / b.W), col = (int)(k - (size_t)row * b.W);
+  double a = -1.0;
+  if (col >= b.firstpix && col <= b.lastpix)
+    a = pixel_azoff(b.coef + 3 * row, col, b.firstpix, b.lastpix, b.A, b.L);
+  azoff[k] = a;
+}
+
+// Pairs per line: one block per line.
+__global__ void k_pairs(BurstDev b, int64_t* pairs) {
+  __shared__ int64_t part[kBlock / kWave];
+  const int row = blockIdx.x;
+  int64_t acc = 0;
+  if (row < b.nfit) {
+    for (int col = b.firstpix + threadIdx.x; col <= b.lastpix && col < b.W;
+         col += blockDim.x) {
+      const double az =
+          pixel_azoff(b.coef + 3 * row, col, b.firstpix, b.lastpix, b.A, b.L);
+      if (az > 0.) {
+        int s, e;
+        pulse_window(az, b.A, b.L, &s, &e);
+        if (e > s) acc += e - s;
+      }
+    }
+  }
+  for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
+  if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int64_t t = 0;
+    for (int w = 0; w < kBlock / kWave; ++w) t += part[w];
+    pairs[row] = t;
+  }
+}
+
+// pixelint on materialised xyz / azoff: thread per pixel, FP64 like k_plain.
+__global__ void k_pixelint(BurstDev b, const double* __restrict__ xyz,
+                           const double* __restrict__ azoff) {
+  const size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t total = (size_t)b.nlines * b.W;
+  if (k >= total) return;
+  double ar = 0., ai = 0.;
+  int azs = 0, aze = 0;
+  const double az = azoff[k];
+  if (az > 0.) {
+    pulse_window(az, b.A, b.L, &azs, &aze);
+    const double x[3] = {xyz[3 * k], xyz[3 * k + 1], xyz[3 * k + 2]};
+    const float2* base = (const float2*)b.data;
+    for (int q = azs; q < aze; ++q)
+      plain_pair(base + (size_t)q * b.S, x, b.satloc[3 * q],
+                 b.satloc[3 * q + 1], b.satloc[3 * q + 2], b, ar, ai);
+  }
+  float2 o = make_float2(0.f, 0.f);
+  if (az > 0.) {
+    const float s = (float)(1. / (aze - azs));
+    o = make_float2((float)ar * s, (float)ai * s);
+  }
+  b.out[k] = o;
+}
+
+// ------------------------------------------------------------ utilities
+
+__global__ void k_zero16(uint4* p, size_t n16, uint8_t* tail, int ntail) {
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n16;
+       i += stride)
+    p[i] = make_uint4(0u, 0u, 0u, 0u);
+  if (blockIdx.x == 0 && (int)threadIdx.x < ntail) tail[threadIdx.x] = 0;
+}
+
+__global__ void k_zero1(uint8_t* p, size_t n) {
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += stride)
+    p[i] = 0;
+}
+
+// One block per pulse: max |component|, then the packed row.
+__global__ void k_pack(const float2* __restrict__ in, int S, int dtype,
+                       uint32_t* __restrict__ out, float* __restrict__ scale) {
+  __shared__ float part[kBlock / kWave];
+  const int p = blockIdx.x;
+  const float2* row = in + (size_t)p * S;
+  float m = 0.f;
+  for (int i = threadIdx.x; i < S; i += blockDim.x) {
+    const float2 v = row[i];
+    m = fmaxf(m, fmaxf(fabsf(v.x), fabsf(v.y)));
+  }
+  for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o, 64));
+  if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = m;
+  __syncthreads();
+  float mx = part[0];
+  for (int w = 1; w < kBlock / kWave; ++w) mx = fmaxf(mx, part[w]);
+  const float sc = mx > 0.f ? mx * (1.0f / 16384.0f) : 1.0f;
+  const float inv = 1.0f / sc;
+  if (threadIdx.x == 0) scale[p] = sc;
+  uint32_t* orow = out + (size_t)p * S;
+  for (int i = threadIdx.x; i < S; i += blockDim.x) {
+    const float2 v = row[i];
+    const float re = v.x * inv, im = v.y * inv;
+    uint32_t w;
+    if (dtype == TDBP_CF16) {
+      typedef _Float16 h2 __attribute__((ext_vector_type(2)));
+      h2 h;
+      h.x = (_Float16)re;
+      h.y = (_Float16)im;
+      __builtin_memcpy(&w, &h, 4);
+    } else {
+      // round to nearest even into the upper 16 bits
+      uint32_t a = __float_as_uint(re), c = __float_as_uint(im);
+      a = (a + 0x7fffu + ((a >> 16) & 1u)) >> 16;
+      c = (c + 0x7fffu + ((c >> 16) & 1u)) >> 16;
+      w = (a & 0xffffu) | (c << 16);
+    }
+    orow[i] = w;
+  }
+}
+
+// Counter-based normal pair (splitmix64 hash -> Box-Muller).
+__device__ __forceinline__ uint64_t mix64(uint64_t z) {
+  z += 0x9e3779b97f4a7c15ull;
+  z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+  z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+  return z ^ (z >> 31);
+}
+__device__ __forceinline__ float2 gauss2(uint64_t seed, uint64_t k) {
+  const uint64_t h = mix64(seed ^ mix64(k));
+  const float u1 = ((uint32_t)(h >> 40) + 0.5f) * (1.0f / 16777216.0f);
+  const float u2 = ((uint32_t)(h & 0xffffffu) + 0.5f) * (1.0f / 16777216.0f);
+  const float r = sqrtf(-2.0f * logf(u1));
+  float s, c;
+  sincospif(2.0f * u2, &s, &c);
+  return make_float2(r * c, r * s);
+}
+
+struct Taps {
+  float h[64];
+};
+
+__global__ void k_synth_clutter(float2* d, int L, int S, uint64_t seed,
+                                float sigma, Taps taps, int ntaps) {
+  const size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= (size_t)L * S) return;
+  const int p = (int)(k / S), s = (int)(k - (size_t)p * S);
+  const int half = ntaps / 2;
+  float re = 0.f, im = 0.f;
+  const float g = sigma * 0.70710678118654752f;
+  for (int t = 0; t < ntaps; ++t) {
+    const int j = s - t + half;
+    if (j < 0 || j >= S) continue;
+    const float2 n = gauss2(seed, (uint64_t)p * (uint64_t)S + (uint64_t)j);
+    re = fmaf(taps.h[t], n.x, re);
+    im = fmaf(taps.h[t], n.y, im);
+  }
+  d[k] = make_float2(g * re, g * im);
+}
+
+__global__ void k_synth_target(float2* d, BurstDev b, double tx, double ty,
+                               double tz, float ar, float ai, float res,
+                               int halfwidth) {
+  const int p = blockIdx.x;
+  const int j = (int)threadIdx.x - halfwidth;
+  if (p >= b.L || j > halfwidth) return;
+  const double* s = b.satloc + 3 * p;
+  const double dx = tx - s[0], dy = ty - s[1], dz = tz - s[2];
+  const double R = sqrt(dx * dx + dy * dy + dz * dz);
+  const double r = (R - b.rngstart) * b.inv_dmrg;
+  const int i = (int)rint(r) + j;
+  if (i < 0 || i >= b.S) return;
+  const double cyc = R * b.two_over_wvl;
+  const double f = cyc - rint(cyc);
+  double sn, cs;
+  sincospi(2.0 * f, &sn, &cs);
+  const double xarg = (i - r) / res;
+  const double snc = fabs(xarg) < 1e-12 ? 1.0 : sinpi(xarg) / (kPi * xarg);
+  // amplitude * sinc * exp(-j 2 pi f)
+  const double er = (ar * cs + ai * sn) * snc;
+  const double ei = (ai * cs - ar * sn) * snc;
+  float2* o = d + (size_t)p * b.S + i;
+  o->x += (float)er;
+  o->y += (float)ei;
+}
+
+// ------------------------------------------------------------ launchers
+
+namespace {
+inline unsigned blocks_for(size_t n, int bs) {
+  return (unsigned)((n + bs - 1) / bs);
+}
+
+template <int TR, bool STACK>
+hipError_t plain_launch(const BurstDev& b, hipStream_t s) {
+  const int nb = b.tiles_r * b.tiles_c;
+  hipLaunchKernelGGL((k_plain<TR, STACK>), dim3(nb), dim3(kBlock), 0, s, b);
+  return hipGetLastError();
+}
+
+template <int TR, bool STACK, int DT>
+hipError_t mixed_launch(const BurstDev& b, hipStream_t s) {
+  const int nb = b.tiles_r * b.tiles_c;
+  hipLaunchKernelGGL((k_mixed<TR, STACK, DT>), dim3(nb), dim3(kBlock), 0, s, b);
+  return hipGetLastError();
+}
+
+template <int TR, bool STACK>
+hipError_t mixed_dt(const BurstDev& b, hipStream_t s) {
+  switch (b.dtype) {
+    case TDBP_CF32: return mixed_launch<TR, STACK, TDBP_CF32>(b, s);
+    case TDBP_CF16: return mixed_launch<TR, STACK, TDBP_CF16>(b, s);
+    default: return mixed_launch<TR, STACK, TDBP_CBF16>(b, s);
+  }
+}
+
+template <int PREC, int TR, bool STACK, int DT>
+hipError_t batch_launch(const BurstDev* d, const int2* it, const int* qb,
+                        int* ctr, int grid, hipStream_t s) {
+  hipLaunchKernelGGL((k_batch<PREC, TR, STACK, DT>), dim3(grid), dim3(kBlock),
+                     0, s, d, it, qb, ctr);
+  return hipGetLastError();
+}
+
+template <int PREC, int TR, bool STACK, int DT>
+int occupancy() {
+  int n = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
+          &n, k_batch<PREC, TR, STACK, DT>, kBlock, 0) != hipSuccess)
+    return 1;
+  return n > 0 ? n : 1;
+}
+
+// Dispatch (precision, variant, dtype) to a template instance.
+template <template <int, int, bool, int> class F, typename... Args>
+auto dispatch(int prec, int variant, int dt, Args... args) {
+#define TDBP_V(TR, ST)                                                     \
+  if (prec == TDBP_PLAIN_FP64) return F<TDBP_PLAIN_FP64, TR, ST, TDBP_CF32>::run(args...); \
+  if (dt == TDBP_CF32) return F<TDBP_MIXED, TR, ST, TDBP_CF32>::run(args...); \
+  if (dt == TDBP_CF16) return F<TDBP_MIXED, TR, ST, TDBP_CF16>::run(args...); \
+  return F<TDBP_MIXED, TR, ST, TDBP_CBF16>::run(args...);
+  switch (variant) {
+    case 0: { TDBP_V(8, false) }
+    case 1: { TDBP_V(4, false) }
+    case 2: { TDBP_V(2, false) }
+    case 3: { TDBP_V(1, false) }
+    case 4: { TDBP_V(8, true) }
+    case 5: { TDBP_V(4, true) }
+    case 6: { TDBP_V(2, true) }
+    default: { TDBP_V(1, true) }
+  }
+#undef TDBP_V
+}
+
+template <int PREC, int TR, bool STACK, int DT>
+struct BatchRun {
+  static hipError_t run(const BurstDev* d, const int2* it, const int* qb,
+                        int* ctr, int grid, hipStream_t s) {
+    return batch_launch<PREC, TR, STACK, DT>(d, it, qb, ctr, grid, s);
+  }
+};
+template <int PREC, int TR, bool STACK, int DT>
+struct OccRun {
+  static int run() { return occupancy<PREC, TR, STACK, DT>(); }
+};
+}  // namespace
+
+hipError_t launch_plain(const BurstDev& b, int variant, hipStream_t s) {
+  switch (variant) {
+    case 0: return plain_launch<8, false>(b, s);
+    case 1: return plain_launch<4, false>(b, s);
+    case 2: return plain_launch<2, false>(b, s);
+    case 3: return plain_launch<1, false>(b, s);
+    case 4: return plain_launch<8, true>(b, s);
+    case 5: return plain_launch<4, true>(b, s);
+    case 6: return plain_launch<2, true>(b, s);
+    default: return plain_launch<1, true>(b, s);
+  }
+}
+
+hipError_t launch_mixed(const BurstDev& b, int variant, hipStream_t s) {
+  switch (variant) {
+    case 0: return mixed_dt<8, false>(b, s);
+    case 1: return mixed_dt<4, false>(b, s);
+    case 2: return mixed_dt<2, false>(b, s);
+    case 3: return mixed_dt<1, false>(b, s);
+    case 4: return mixed_dt<8, true>(b, s);
+    case 5: return mixed_dt<4, true>(b, s);
+    case 6: return mixed_dt<2, true>(b, s);
+    default: return mixed_dt<1, true>(b, s);
+  }
+}
+
+int batch_blocks_per_cu(int precision, int variant, int dtype) {
+  return dispatch<OccRun>(precision, variant, dtype);
+}
+
+hipError_t launch_batch(const BurstDev* d_bursts, const int2* d_items,
+                        const int* d_qbase, int* d_counters, int precision,
+                        int variant, int dtype, int grid, hipStream_t s) {
+  return dispatch<BatchRun>(precision, variant, dtype, d_bursts, d_items,
+                            d_qbase, d_counters, grid, s);
+}
+
+hipError_t launch_batch_fit(const BurstDev* d_bursts, int nbursts,
+                            int max_lines, hipStream_t s) {
+  dim3 grid(blocks_for(max_lines, 64), nbursts);
+  hipLaunchKernelGGL(k_batch_fit, grid, dim3(64), 0, s, d_bursts);
+  return hipGetLastError();
+}
+
+hipError_t launch_fit_one(const BurstDev& b, double* coef, hipStream_t s) {
+  hipLaunchKernelGGL(k_fit, dim3(blocks_for(b.nlines, 64)), dim3(64), 0, s, b,
+                     coef);
+  return hipGetLastError();
+}
+
+hipError_t launch_setxyz(const BurstDev& b, double* xyz, double* xyzfit,
+                         hipStream_t s) {
+  const size_t n = (size_t)b.nlines * b.W;
+  hipLaunchKernelGGL(k_setxyz, dim3(blocks_for(n, kBlock)), dim3(kBlock), 0, s,
+                     b, xyz, xyzfit);
+  return hipGetLastError();
+}
+
+hipError_t launch_setazoff(const BurstDev& b, double* azoff, hipStream_t s) {
+  const size_t n = (size_t)b.nlines * b.W;
+  hipLaunchKernelGGL(k_setazoff, dim3(blocks_for(n, kBlock)), dim3(kBlock), 0,
+                     s, b, azoff);
+  return hipGetLastError();
+}
+
+hipError_t launch_pair_count(const BurstDev& b, int64_t* pairs, hipStream_t s) {
+  hipLaunchKernelGGL(k_pairs, dim3(b.nlines), dim3(kBlock), 0, s, b, pairs);
+  return hipGetLastError();
+}
+
+hipError_t launch_pixelint(const BurstDev& b, const double* xyz,
+                           const double* azoff, hipStream_t s) {
+  const size_t n = (size_t)b.nlines * b.W;
+  hipLaunchKernelGGL(k_pixelint, dim3(blocks_for(n, kBlock)), dim3(kBlock), 0,
+                     s, b, xyz, azoff);
+  return hipGetLastError();
+}
+
+hipError_t launch_zero(void* p, size_t bytes, hipStream_t s) {
+  if (bytes == 0) return hipSuccess;
+  if ((uintptr_t)p % 16 != 0) {
+    size_t g = blocks_for(bytes, kBlock);
+    hipLaunchKernelGGL(k_zero1, dim3((unsigned)(g < 65535 ? g : 65535)),
+                       dim3(kBlock), 0, s, (uint8_t*)p, bytes);
+    return hipGetLastError();
+  }
+  const size_t n16 = bytes / 16;
+  const int ntail = (int)(bytes - n16 * 16);
+  size_t g = blocks_for(n16, kBlock);
+  g = g == 0 ? 1 : (g < 65535 ? g : 65535);
+  hipLaunchKernelGGL(k_zero16, dim3((unsigned)g), dim3(kBlock), 0, s,
+                     (uint4*)p, n16, (uint8_t*)p + n16 * 16, ntail);
+  return hipGetLastError();
+}
+
+hipError_t launch_pack(const float2* in, int L, int S, int dtype, void* out,
+                       float* scale, hipStream_t s) {
+  hipLaunchKernelGGL(k_pack, dim3(L), dim3(kBlock), 0, s, in, S, dtype,
+                     (uint32_t*)out, scale);
+  return hipGetLastError();
+}
+
+hipError_t launch_synth_clutter(float2* d, int L, int S, uint64_t seed,
+                                float sigma, const float* taps, int ntaps,
+                                hipStream_t s) {
+  Taps t;
+  for (int i = 0; i < 64; ++i) t.h[i] = i < ntaps ? taps[i] : 0.f;
+  const size_t n = (size_t)L * S;
+  hipLaunchKernelGGL(k_synth_clutter, dim3(blocks_for(n, kBlock)), dim3(kBlock),
+                     0, s, d, L, S, seed, sigma, t, ntaps);
+  return hipGetLastError();
+}
+
+hipError_t launch_synth_target(float2* d, const BurstDev& b, double tx,
+                               double ty, double tz, float ar, float ai,
+                               float res, int halfwidth, hipStream_t s) {
+  hipLaunchKernelGGL(k_synth_target, dim3(b.L), dim3(2 * halfwidth + 1), 0, s,
+                     d, b, tx, ty, tz, ar, ai, res, halfwidth);
+  return hipGetLastError();
+}
+
+}  // namespace tdbp

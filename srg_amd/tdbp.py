@@ -1,0 +1,506 @@
+"""ctypes binding of libsrg_tdbp.so (include/tdbp.h) and thin host helpers.
+
+The library is the product: every compute call here goes to HIP kernels on
+the GPU.  There is no CPU fallback -- if the library is missing, or no gfx950
+device is visible, the calls raise.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass, field
+from typing import Optional
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libsrg_tdbp.so")
+
+# tdbp_status
+OK, ERR_ARG, ERR_HIP, ERR_NODEV, ERR_IO, ERR_NOMEM, ERR_LIMIT = range(7)
+# tdbp_precision
+PLAIN_FP64, MIXED = 0, 1
+# tdbp_dtype
+CF32, CF16, CBF16 = 0, 1, 2
+DTYPE_NAMES = {CF32: "cf32", CF16: "cf16", CBF16: "cbf16"}
+
+
+class TdbpError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"tdbp error {code}: {msg}")
+        self.code = code
+
+
+class Geom(C.Structure):
+    """Mirror of tdbp_geom."""
+
+    _fields_ = [
+        ("rawdatalines", C.c_int32),
+        ("samples", C.c_int32),
+        ("aperture", C.c_int32),
+        ("numstatevec", C.c_int32),
+        ("rngstart", C.c_double),
+        ("rngend", C.c_double),
+        ("dmrg", C.c_double),
+        ("wvl", C.c_double),
+        ("prf", C.c_double),
+        ("angc0", C.c_double),
+        ("angc1", C.c_double),
+        ("tmid", C.c_double),
+        ("xyz_mid", C.c_double * 3),
+        ("vel_mid", C.c_double * 3),
+        ("timeorbit", C.POINTER(C.c_double)),
+        ("xx", C.POINTER(C.c_double)),
+        ("vv", C.POINTER(C.c_double)),
+        ("demwidth", C.c_int32),
+        ("firstline", C.c_int32),
+        ("nlines", C.c_int32),
+        ("firstpix", C.c_int32),
+        ("lastpix", C.c_int32),
+        ("firstlat", C.c_double),
+        ("firstlon", C.c_double),
+        ("deltalat", C.c_double),
+        ("deltalon", C.c_double),
+        ("fit_last_line", C.c_int32),
+    ]
+
+
+class Samples(C.Structure):
+    """Mirror of tdbp_samples."""
+
+    _fields_ = [("d_data", C.c_void_p), ("d_scale", C.c_void_p), ("dtype", C.c_int32)]
+
+
+_lib = None
+
+_VP = C.c_void_p
+_DP = C.POINTER(C.c_double)
+_IP = C.POINTER(C.c_int)
+_GP = C.POINTER(Geom)
+_SP = C.POINTER(Samples)
+
+
+def _sig(lib):
+    sigs = {
+        "tdbp_last_error": (C.c_char_p, []),
+        "tdbp_device_count": (C.c_int, []),
+        "tdbp_init": (C.c_int, [C.c_int]),
+        "tdbp_dropin_status": (C.c_int, []),
+        "tdbp_dev_alloc": (_VP, [C.c_size_t]),
+        "tdbp_dev_free": (C.c_int, [_VP]),
+        "tdbp_memcpy_htod": (C.c_int, [_VP, _VP, C.c_size_t, _VP]),
+        "tdbp_memcpy_dtoh": (C.c_int, [_VP, _VP, C.c_size_t, _VP]),
+        "tdbp_memcpy_dtod": (C.c_int, [_VP, _VP, C.c_size_t, _VP]),
+        "tdbp_dev_zero": (C.c_int, [_VP, C.c_size_t, _VP]),
+        "tdbp_stream_create": (C.c_int, [C.POINTER(_VP)]),
+        "tdbp_stream_destroy": (C.c_int, [_VP]),
+        "tdbp_stream_sync": (C.c_int, [_VP]),
+        "tdbp_device_sync": (C.c_int, []),
+        "tdbp_event_create": (C.c_int, [C.POINTER(_VP)]),
+        "tdbp_event_destroy": (C.c_int, [_VP]),
+        "tdbp_event_record": (C.c_int, [_VP, _VP]),
+        "tdbp_event_elapsed_ms": (C.c_int, [_VP, _VP, C.POINTER(C.c_float)]),
+        "tdbp_mem_info": (C.c_int, [C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)]),
+        "tdbp_window": (None, [_DP, C.c_double, C.c_double, C.c_double, C.c_double,
+                               C.c_int, C.c_int, _IP, _IP, _IP, _IP]),
+        "tdbp_orbit_interp": (C.c_int, [_DP, _DP, _DP, C.c_int, _DP, C.c_int, _DP, _DP]),
+        "tdbp_setxyz": (C.c_int, [_GP, _VP, _VP, _VP, _VP]),
+        "tdbp_fit_aperture": (C.c_int, [_GP, _VP, _VP, _VP]),
+        "tdbp_setazoff": (C.c_int, [_GP, _VP, _VP, _VP]),
+        "tdbp_pair_count": (C.c_int, [_GP, _VP, _VP, _VP]),
+        "tdbp_pack_samples": (C.c_int, [_VP, C.c_int, C.c_int, C.c_int, _VP, _VP, _VP]),
+        "tdbp_pixelint": (C.c_int, [_GP, _VP, _VP, _VP, _VP, _VP, _VP]),
+        "tdbp_backproject": (C.c_int, [_GP, _SP, _VP, _VP, _VP, _VP, C.c_int, _VP]),
+        "tdbp_backproject_variant": (C.c_int, [_GP, _SP, _VP, _VP, _VP, _VP, C.c_int,
+                                               C.c_int, _VP]),
+        "tdbp_autotune": (C.c_int, [_GP, _SP, _VP, _VP, _VP, C.c_int, C.c_int,
+                                    C.POINTER(C.c_float), _IP]),
+        "tdbp_set_variant": (C.c_int, [C.c_int]),
+        "tdbp_variant_count": (C.c_int, []),
+        "tdbp_variant_info": (C.c_int, [C.c_int, _IP, _IP, _IP]),
+        "tdbp_batch_create": (_VP, [C.c_int, C.c_int]),
+        "tdbp_batch_add": (C.c_int, [_VP, _GP, _SP, _VP, _VP, _VP]),
+        "tdbp_batch_run": (C.c_int, [_VP, _VP]),
+        "tdbp_batch_pairs": (C.c_int64, [_VP]),
+        "tdbp_batch_variant": (C.c_int, [_VP]),
+        "tdbp_batch_device_bytes": (C.c_size_t, [_VP]),
+        "tdbp_batch_destroy": (None, [_VP]),
+        "tdbp_synth_clutter": (C.c_int, [_VP, C.c_int, C.c_int, C.c_uint64, C.c_float,
+                                         C.POINTER(C.c_float), C.c_int, _VP]),
+        "tdbp_synth_targets": (C.c_int, [_VP, _GP, _VP, _DP, C.POINTER(C.c_float),
+                                         C.c_int, C.c_float, C.c_int, _VP]),
+        "azimuth_compress_": (None, [_VP, _DP, _IP, _IP, _IP, _IP, _IP, _IP, _DP, _DP,
+                                     _DP, _DP, _DP, _DP, _DP, _DP, _IP, _DP, _DP, _DP,
+                                     _DP, _DP, _DP, _DP, _DP, _DP, _DP, _DP, _IP, _IP,
+                                     _DP, _DP, _DP]),
+    }
+    for name, (res, args) in sigs.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    return lib
+
+
+def exported_symbols():
+    """Names of the entry points the binding declares (all must exist)."""
+    return [
+        "tdbp_last_error", "tdbp_device_count", "tdbp_init", "tdbp_dropin_status",
+        "tdbp_dev_alloc", "tdbp_dev_free", "tdbp_memcpy_htod", "tdbp_memcpy_dtoh",
+        "tdbp_memcpy_dtod", "tdbp_dev_zero", "tdbp_stream_create", "tdbp_stream_destroy",
+        "tdbp_stream_sync", "tdbp_device_sync", "tdbp_event_create", "tdbp_event_destroy",
+        "tdbp_event_record", "tdbp_event_elapsed_ms", "tdbp_mem_info", "tdbp_window",
+        "tdbp_orbit_interp", "tdbp_setxyz", "tdbp_fit_aperture", "tdbp_setazoff",
+        "tdbp_pair_count", "tdbp_pack_samples", "tdbp_pixelint", "tdbp_backproject",
+        "tdbp_backproject_variant", "tdbp_autotune", "tdbp_set_variant",
+        "tdbp_variant_count", "tdbp_variant_info", "tdbp_batch_create", "tdbp_batch_add",
+        "tdbp_batch_run", "tdbp_batch_pairs", "tdbp_batch_variant",
+        "tdbp_batch_device_bytes", "tdbp_batch_destroy", "tdbp_synth_clutter",
+        "tdbp_synth_targets", "azimuth_compress_",
+    ]
+
+
+def lib():
+    """Load the HIP library (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise TdbpError(ERR_NODEV, f"{LIB_PATH} not built (run __graft_entry__.build())")
+        _lib = _sig(C.CDLL(LIB_PATH))
+    return _lib
+
+
+def check(code: int):
+    if code != OK:
+        raise TdbpError(code, lib().tdbp_last_error().decode(errors="replace"))
+
+
+def init(device: int = 0):
+    check(lib().tdbp_init(device))
+
+
+def _dp(a: np.ndarray):
+    return a.ctypes.data_as(_DP)
+
+
+# ----------------------------------------------------------------- memory
+
+
+class DeviceArray:
+    """A device allocation with a shape and dtype (numpy-like metadata)."""
+
+    def __init__(self, shape, dtype):
+        self.shape = tuple(int(s) for s in np.atleast_1d(shape))
+        self.dtype = np.dtype(dtype)
+        self.nbytes = int(np.prod(self.shape, dtype=np.int64)) * self.dtype.itemsize
+        p = lib().tdbp_dev_alloc(self.nbytes)
+        if not p:
+            check(ERR_NOMEM)
+        self.ptr = p
+
+    @classmethod
+    def from_host(cls, a: np.ndarray, stream=None):
+        a = np.ascontiguousarray(a)
+        d = cls(a.shape, a.dtype)
+        check(lib().tdbp_memcpy_htod(d.ptr, a.ctypes.data, a.nbytes, stream))
+        return d
+
+    def upload(self, a: np.ndarray, stream=None):
+        a = np.ascontiguousarray(a, dtype=self.dtype)
+        assert a.nbytes == self.nbytes
+        check(lib().tdbp_memcpy_htod(self.ptr, a.ctypes.data, a.nbytes, stream))
+
+    def download(self, stream=None) -> np.ndarray:
+        out = np.empty(self.shape, self.dtype)
+        check(lib().tdbp_memcpy_dtoh(out.ctypes.data, self.ptr, self.nbytes, stream))
+        return out
+
+    def zero(self, stream=None):
+        check(lib().tdbp_dev_zero(self.ptr, self.nbytes, stream))
+
+    def free(self):
+        if self.ptr:
+            lib().tdbp_dev_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class Stream:
+    def __init__(self):
+        s = C.c_void_p()
+        check(lib().tdbp_stream_create(C.byref(s)))
+        self.ptr = s.value
+
+    def sync(self):
+        check(lib().tdbp_stream_sync(self.ptr))
+
+    def __del__(self):
+        try:
+            if self.ptr:
+                lib().tdbp_stream_destroy(self.ptr)
+        except Exception:
+            pass
+
+
+class Event:
+    def __init__(self):
+        e = C.c_void_p()
+        check(lib().tdbp_event_create(C.byref(e)))
+        self.ptr = e.value
+
+    def record(self, stream=None):
+        check(lib().tdbp_event_record(self.ptr, getattr(stream, "ptr", stream)))
+
+    def elapsed_ms(self, later: "Event") -> float:
+        ms = C.c_float()
+        check(lib().tdbp_event_elapsed_ms(self.ptr, later.ptr, C.byref(ms)))
+        return ms.value
+
+    def __del__(self):
+        try:
+            if self.ptr:
+                lib().tdbp_event_destroy(self.ptr)
+        except Exception:
+            pass
+
+
+def sync():
+    check(lib().tdbp_device_sync())
+
+
+def mem_info():
+    f, t = C.c_size_t(), C.c_size_t()
+    check(lib().tdbp_mem_info(C.byref(f), C.byref(t)))
+    return f.value, t.value
+
+
+# ----------------------------------------------------------------- geometry
+
+
+@dataclass
+class BurstGeometry:
+    """Host description of one burst and its DEM window (see tdbp_geom)."""
+
+    L: int
+    S: int
+    A: int
+    rngstart: float
+    rngend: float
+    dmrg: float
+    wvl: float
+    prf: float
+    angc0: float
+    angc1: float
+    tmid: float
+    xyz_mid: np.ndarray
+    vel_mid: np.ndarray
+    timeorbit: np.ndarray
+    xx: np.ndarray
+    vv: np.ndarray
+    demwidth: int
+    firstline: int
+    nlines: int
+    firstpix: int
+    lastpix: int
+    firstlat: float
+    firstlon: float
+    deltalat: float
+    deltalon: float
+    fit_last_line: int = 0
+    _keep: list = field(default_factory=list, repr=False)
+
+    def to_c(self) -> Geom:
+        t = np.ascontiguousarray(self.timeorbit, np.float64)
+        x = np.ascontiguousarray(self.xx, np.float64)
+        v = np.ascontiguousarray(self.vv, np.float64)
+        self._keep = [t, x, v]
+        g = Geom()
+        g.rawdatalines, g.samples, g.aperture = self.L, self.S, self.A
+        g.numstatevec = len(t)
+        g.rngstart, g.rngend, g.dmrg, g.wvl = self.rngstart, self.rngend, self.dmrg, self.wvl
+        g.prf, g.angc0, g.angc1, g.tmid = self.prf, self.angc0, self.angc1, self.tmid
+        for i in range(3):
+            g.xyz_mid[i] = float(self.xyz_mid[i])
+            g.vel_mid[i] = float(self.vel_mid[i])
+        g.timeorbit, g.xx, g.vv = _dp(t), _dp(x), _dp(v)
+        g.demwidth, g.firstline, g.nlines = self.demwidth, self.firstline, self.nlines
+        g.firstpix, g.lastpix = self.firstpix, self.lastpix
+        g.firstlat, g.firstlon = self.firstlat, self.firstlon
+        g.deltalat, g.deltalon = self.deltalat, self.deltalon
+        g.fit_last_line = self.fit_last_line
+        return g
+
+    @property
+    def nfit(self) -> int:
+        return self.nlines if self.fit_last_line else self.nlines - 1
+
+
+def window(latlons, firstlat, firstlon, deltalat, deltalon, demwidth, demlength):
+    ll = np.ascontiguousarray(latlons, np.float64)
+    out = [C.c_int() for _ in range(4)]
+    lib().tdbp_window(_dp(ll), firstlat, firstlon, deltalat, deltalon, demwidth,
+                      demlength, *[C.byref(o) for o in out])
+    return tuple(o.value for o in out)
+
+
+def orbit_interp(timeorbit, xx, vv, times):
+    """Hermite positions and velocities at `times` (host code of the library)."""
+    t = np.ascontiguousarray(timeorbit, np.float64)
+    x = np.ascontiguousarray(xx, np.float64)
+    v = np.ascontiguousarray(vv, np.float64)
+    tm = np.ascontiguousarray(np.atleast_1d(times), np.float64)
+    pos = np.empty((len(tm), 3))
+    vel = np.empty((len(tm), 3))
+    check(lib().tdbp_orbit_interp(_dp(t), _dp(x), _dp(v), len(t), _dp(tm), len(tm),
+                                  _dp(pos), _dp(vel)))
+    return pos, vel
+
+
+# ------------------------------------------------------------ operators
+
+
+def _s(stream):
+    return getattr(stream, "ptr", stream)
+
+
+def samples(d_data: DeviceArray, dtype=CF32, d_scale: Optional[DeviceArray] = None):
+    s = Samples()
+    s.d_data = d_data.ptr
+    s.d_scale = d_scale.ptr if d_scale is not None else None
+    s.dtype = dtype
+    return s
+
+
+def fit_aperture(g: BurstGeometry, d_dem: DeviceArray, d_coef: DeviceArray, stream=None):
+    gc = g.to_c()
+    check(lib().tdbp_fit_aperture(C.byref(gc), d_dem.ptr, d_coef.ptr, _s(stream)))
+
+
+def setxyz(g: BurstGeometry, d_dem, d_xyz, d_xyzfit=None, stream=None):
+    gc = g.to_c()
+    check(lib().tdbp_setxyz(C.byref(gc), d_dem.ptr, d_xyz.ptr,
+                            d_xyzfit.ptr if d_xyzfit is not None else None, _s(stream)))
+
+
+def setazoff(g: BurstGeometry, d_coef, d_azoff, stream=None):
+    gc = g.to_c()
+    check(lib().tdbp_setazoff(C.byref(gc), d_coef.ptr, d_azoff.ptr, _s(stream)))
+
+
+def pair_count(g: BurstGeometry, d_coef, d_pairs, stream=None):
+    gc = g.to_c()
+    check(lib().tdbp_pair_count(C.byref(gc), d_coef.ptr, d_pairs.ptr, _s(stream)))
+
+
+def pixelint(g: BurstGeometry, d_burst, d_satloc, d_xyz, d_azoff, d_out, stream=None):
+    gc = g.to_c()
+    check(lib().tdbp_pixelint(C.byref(gc), d_burst.ptr, d_satloc.ptr, d_xyz.ptr,
+                              d_azoff.ptr, d_out.ptr, _s(stream)))
+
+
+def pack(d_cf32: DeviceArray, L: int, S: int, dtype: int, stream=None):
+    d_packed = DeviceArray((L, S), np.uint32)
+    d_scale = DeviceArray((L,), np.float32)
+    check(lib().tdbp_pack_samples(d_cf32.ptr, L, S, dtype, d_packed.ptr, d_scale.ptr,
+                                  _s(stream)))
+    return d_packed, d_scale
+
+
+def backproject(g: BurstGeometry, smp: Samples, d_satloc, d_dem, d_out,
+                precision=PLAIN_FP64, d_coef=None, variant=None, stream=None):
+    gc = g.to_c()
+    coef = d_coef.ptr if d_coef is not None else None
+    if variant is None:
+        check(lib().tdbp_backproject(C.byref(gc), C.byref(smp), d_satloc.ptr, d_dem.ptr,
+                                     coef, d_out.ptr, precision, _s(stream)))
+    else:
+        check(lib().tdbp_backproject_variant(C.byref(gc), C.byref(smp), d_satloc.ptr,
+                                             d_dem.ptr, coef, d_out.ptr, precision,
+                                             variant, _s(stream)))
+
+
+def autotune(g: BurstGeometry, smp: Samples, d_satloc, d_dem, d_out, precision=MIXED,
+             reps=3):
+    gc = g.to_c()
+    n = lib().tdbp_variant_count()
+    ms = (C.c_float * n)()
+    best = C.c_int(-1)
+    check(lib().tdbp_autotune(C.byref(gc), C.byref(smp), d_satloc.ptr, d_dem.ptr,
+                              d_out.ptr, precision, reps, ms, C.byref(best)))
+    return best.value, list(ms)
+
+
+def variant_info(v: int):
+    r, c, s = C.c_int(), C.c_int(), C.c_int()
+    check(lib().tdbp_variant_info(v, C.byref(r), C.byref(c), C.byref(s)))
+    return r.value, c.value, s.value
+
+
+class Batch:
+    """Many bursts in one persistent launch (tdbp_batch_*)."""
+
+    def __init__(self, precision=MIXED, variant=-1):
+        p = lib().tdbp_batch_create(precision, variant)
+        if not p:
+            check(ERR_ARG)
+        self.ptr = p
+        self._keep = []
+
+    def add(self, g: BurstGeometry, smp: Samples, d_satloc, d_dem, d_out) -> int:
+        gc = g.to_c()
+        self._keep.append((g, smp, d_satloc, d_dem, d_out))
+        r = lib().tdbp_batch_add(self.ptr, C.byref(gc), C.byref(smp), d_satloc.ptr,
+                                 d_dem.ptr, d_out.ptr)
+        if r < 0:
+            check(-r)
+        return r
+
+    def run(self, stream=None):
+        check(lib().tdbp_batch_run(self.ptr, _s(stream)))
+
+    @property
+    def pairs(self) -> int:
+        return int(lib().tdbp_batch_pairs(self.ptr))
+
+    @property
+    def variant(self) -> int:
+        return int(lib().tdbp_batch_variant(self.ptr))
+
+    @property
+    def device_bytes(self) -> int:
+        return int(lib().tdbp_batch_device_bytes(self.ptr))
+
+    def close(self):
+        if self.ptr:
+            lib().tdbp_batch_destroy(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def synth_clutter(d: DeviceArray, L: int, S: int, seed: int, sigma: float,
+                  taps: np.ndarray, stream=None):
+    t = np.ascontiguousarray(taps, np.float32)
+    check(lib().tdbp_synth_clutter(d.ptr, L, S, seed, sigma,
+                                   t.ctypes.data_as(C.POINTER(C.c_float)), len(t),
+                                   _s(stream)))
+
+
+def synth_targets(d: DeviceArray, g: BurstGeometry, d_satloc, xyz, amp, res=1.14,
+                  halfwidth=16, stream=None):
+    gc = g.to_c()
+    x = np.ascontiguousarray(xyz, np.float64).reshape(-1, 3)
+    a = np.empty((len(x), 2), np.float32)
+    a[:, 0] = np.real(amp)
+    a[:, 1] = np.imag(amp)
+    check(lib().tdbp_synth_targets(d.ptr, C.byref(gc), d_satloc.ptr, _dp(x),
+                                   a.ctypes.data_as(C.POINTER(C.c_float)), len(x),
+                                   res, halfwidth, _s(stream)))

@@ -1,0 +1,37 @@
+import os
+import sys
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+
+def pytest_configure(config):
+    config.addinivalue_line("markers", "gpu: needs a gfx950 GPU (run on the GPU box)")
+
+
+@pytest.fixture(scope="session")
+def oracle():
+    from oracle import pyoracle
+
+    return pyoracle.Oracle()
+
+
+@pytest.fixture(scope="session")
+def refcpu():
+    from oracle import pyoracle
+
+    if not pyoracle.ref_available():
+        pytest.skip("reference CPU build (oracle/_ref) not present")
+    return pyoracle.RefCPU()
+
+
+@pytest.fixture(scope="session")
+def gpu():
+    """The HIP library on device 0.  Raises (never skips) when unusable."""
+    from srg_amd import tdbp
+
+    tdbp.init(0)
+    return tdbp

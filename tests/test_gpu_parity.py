@@ -1,0 +1,310 @@
+"""GPU parity: the HIP engine against the oracle on the same inputs.
+
+Tolerances (complex outputs, over computed pixels; phase error power-weighted
+as sqrt(sum |o|^2 dphi^2 / sum |o|^2), amplitude error
+sqrt(sum (|g|-|o|)^2 / sum |o|^2)):
+  * integer work (aperture parabolas, pixel windows, pair counts): exact;
+  * FP64 kernel vs the FP64 oracle: phase <= 1e-6 rad, amplitude <= 1e-6
+    (both are FP64 sums with an FP32 lerp; they differ by sin/cos and sqrt
+    rounding only);
+  * mixed precision with cf32 or cf16 storage: phase <= 1 mrad, amplitude
+    <= 1e-4 (the target of the mixed path);
+  * mixed precision with bf16 storage: reported; bounds 5 mrad / 5e-3 (bf16
+    keeps 8 mantissa bits, its rounding alone exceeds 1e-4 in amplitude).
+"""
+import os
+import tempfile
+
+import numpy as np
+import pytest
+
+from oracle import pyoracle
+from srg_amd import synth
+from srg_amd import tdbp as T
+
+pytestmark = pytest.mark.gpu
+
+
+def errors(g, o, mask=None):
+    g = np.asarray(g, np.complex128).ravel()
+    o = np.asarray(o, np.complex128).ravel()
+    if mask is not None:
+        g, o = g[mask.ravel()], o[mask.ravel()]
+    w = np.abs(o) ** 2
+    dphi = np.angle(g * np.conj(o))
+    ph = float(np.sqrt(np.sum(w * dphi ** 2) / np.sum(w)))
+    amp = float(np.sqrt(np.sum((np.abs(g) - np.abs(o)) ** 2) / np.sum(w)))
+    return ph, amp
+
+
+class Dev:
+    """A burst scene uploaded to the device."""
+
+    def __init__(self, sc, data=None):
+        self.sc = sc
+        g = sc.geom
+        self.data_h = synth.host_samples(sc) if data is None else data
+        self.d_data = T.DeviceArray.from_host(self.data_h)
+        self.d_sat = T.DeviceArray.from_host(sc.satloc)
+        self.d_dem = T.DeviceArray.from_host(sc.dem)
+        self.d_out = T.DeviceArray((g.nlines, g.demwidth), np.complex64)
+        self.d_coef = T.DeviceArray((g.nlines, 3), np.float64)
+
+    def fit(self):
+        T.fit_aperture(self.sc.geom, self.d_dem, self.d_coef)
+        return self.d_coef.download()
+
+    def run(self, precision, dtype=T.CF32, variant=None):
+        if dtype == T.CF32:
+            smp = T.samples(self.d_data, T.CF32)
+        else:
+            self.packed = T.pack(self.d_data, self.sc.L, self.sc.S, dtype)
+            smp = T.samples(self.packed[0], dtype, self.packed[1])
+        self.d_out.zero()
+        T.backproject(self.sc.geom, smp, self.d_sat, self.d_dem, self.d_out, precision,
+                      d_coef=self.d_coef, variant=variant)
+        T.sync()
+        return self.d_out.download()
+
+
+@pytest.fixture(scope="module")
+def c1(gpu, oracle):
+    sc = synth.config1()
+    d = Dev(sc)
+    out, acc, pairs, coef, azoff = oracle.backproject(sc.geom, d.data_h, sc.satloc, sc.dem)
+    return d, dict(out=out, acc=acc, pairs=pairs, coef=coef, azoff=azoff)
+
+
+@pytest.fixture(scope="module")
+def fbm_small(gpu, oracle):
+    src = synth.fbm_surface((512, 512), seed=21)
+    sc = synth.make_burst(1200, 6000, 700, 300, dem="fbm", dem_source=src, ntargets=4,
+                          clutter_sigma=0.1, seed=21)
+    d = Dev(sc)
+    out, acc, pairs, coef, azoff = oracle.backproject(sc.geom, d.data_h, sc.satloc, sc.dem)
+    return d, dict(out=out, acc=acc, pairs=pairs, coef=coef, azoff=azoff)
+
+
+@pytest.mark.parametrize("scene", ["c1", "fbm_small"])
+def test_fit_azoff_pairs_exact(scene, request):
+    d, ref = request.getfixturevalue(scene)
+    g = d.sc.geom
+    coef = d.fit()
+    np.testing.assert_array_equal(coef, ref["coef"])
+    d_az = T.DeviceArray((g.nlines, g.demwidth), np.float64)
+    T.setazoff(g, d.d_coef, d_az)
+    np.testing.assert_array_equal(d_az.download(), ref["azoff"])
+    d_pairs = T.DeviceArray((g.nlines,), np.int64)
+    T.pair_count(g, d.d_coef, d_pairs)
+    assert int(d_pairs.download().sum()) == ref["pairs"]
+
+
+def test_setxyz_matches_oracle(c1, oracle):
+    d, _ = c1
+    g = d.sc.geom
+    xo, fo = oracle.setxyz(g, d.sc.dem)
+    d_xyz = T.DeviceArray((g.nlines, g.demwidth, 3), np.float64)
+    d_fit = T.DeviceArray((g.nlines, 9), np.float64)
+    T.setxyz(g, d.d_dem, d_xyz, d_fit)
+    np.testing.assert_allclose(d_xyz.download(), xo, rtol=0, atol=1e-7)
+    np.testing.assert_allclose(d_fit.download(), fo, rtol=0, atol=1e-7)
+
+
+def test_pixelint_operator_matches_oracle(c1, oracle):
+    d, ref = c1
+    g = d.sc.geom
+    xo, _ = oracle.setxyz(g, d.sc.dem)
+    d_xyz = T.DeviceArray.from_host(xo)
+    d_az = T.DeviceArray.from_host(ref["azoff"])
+    T.pixelint(g, d.d_data, d.d_sat, d_xyz, d_az, d.d_out)
+    T.sync()
+    got = d.d_out.download()
+    m = ref["azoff"] > 0
+    ph, amp = errors(got, ref["out"], m)
+    print(f"pixelint operator vs oracle: phase {ph:.3e} rad, amp {amp:.3e}")
+    assert ph <= 1e-6 and amp <= 1e-6
+    assert np.all(got[~m] == 0)
+
+
+@pytest.mark.parametrize("scene", ["c1", "fbm_small"])
+def test_plain_fp64_matches_oracle(scene, request):
+    d, ref = request.getfixturevalue(scene)
+    d.fit()
+    got = d.run(T.PLAIN_FP64)
+    m = ref["azoff"] > 0
+    ph, amp = errors(got, ref["out"], m)
+    print(f"{scene} plain FP64 vs oracle: phase {ph:.3e} rad, amp {amp:.3e}")
+    assert ph <= 1e-6 and amp <= 1e-6
+    assert np.all(got[~m] == 0)
+
+
+BOUNDS = {T.CF32: (1e-3, 1e-4), T.CF16: (1e-3, 1e-4), T.CBF16: (5e-3, 5e-3)}
+
+
+@pytest.mark.parametrize("scene", ["c1", "fbm_small"])
+@pytest.mark.parametrize("dtype", [T.CF32, T.CF16, T.CBF16])
+def test_mixed_matches_oracle(scene, dtype, request):
+    d, ref = request.getfixturevalue(scene)
+    d.fit()
+    got = d.run(T.MIXED, dtype)
+    m = ref["azoff"] > 0
+    ph, amp = errors(got, ref["out"], m)
+    print(f"{scene} mixed {T.DTYPE_NAMES[dtype]} vs oracle: phase {ph*1e3:.4f} mrad, "
+          f"amp {amp:.3e}")
+    pb, ab = BOUNDS[dtype]
+    assert ph <= pb and amp <= ab
+    assert np.all(got[~m] == 0)
+
+
+def test_all_mixed_variants_agree(c1):
+    d, ref = c1
+    d.fit()
+    m = ref["azoff"] > 0
+    for v in range(T.lib().tdbp_variant_count()):
+        got = d.run(T.MIXED, T.CF16, variant=v)
+        ph, amp = errors(got, ref["out"], m)
+        assert ph <= 1e-3 and amp <= 1e-4, (v, ph, amp)
+        got = d.run(T.PLAIN_FP64, variant=v)
+        ph, amp = errors(got, ref["out"], m)
+        assert ph <= 1e-6 and amp <= 1e-6, (v, ph, amp)
+
+
+def test_plain_is_deterministic(fbm_small):
+    d, _ = fbm_small
+    d.fit()
+    a = d.run(T.PLAIN_FP64)
+    b = d.run(T.PLAIN_FP64)
+    assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+
+
+@pytest.mark.parametrize("precision,dtype", [(T.MIXED, T.CF16), (T.PLAIN_FP64, T.CF32)])
+def test_batch_equals_single_bursts(gpu, precision, dtype):
+    """Uneven bursts in one persistent launch (graph-replayed three times)
+    give bit-identical results to one launch per burst."""
+    src = synth.fbm_surface((256, 256), seed=4)
+    scenes = [synth.make_burst(L, S, W, nl, dem="fbm", dem_source=src, seed=40 + i,
+                               ntargets=2, centre_range=830e3 + 20e3 * i, aperture=180)
+              for i, (L, S, W, nl) in enumerate([(400, 3000, 300, 120), (520, 4100, 170, 90),
+                                                 (450, 2500, 260, 150)])]
+    devs = [Dev(sc) for sc in scenes]
+    v = 4
+    singles = []
+    for d in devs:
+        d.fit()
+        singles.append(d.run(precision, dtype, variant=v))
+    b = T.Batch(precision, v)
+    outs = []
+    for d in devs:
+        smp = (T.samples(d.d_data, T.CF32) if dtype == T.CF32
+               else T.samples(d.packed[0], dtype, d.packed[1]))
+        o = T.DeviceArray((d.sc.geom.nlines, d.sc.geom.demwidth), np.complex64)
+        o.zero()
+        outs.append(o)
+        b.add(d.sc.geom, smp, d.d_sat, d.d_dem, o)
+    for rep in range(3):
+        for o in outs:
+            o.zero()
+        T.sync()
+        b.run()
+        T.sync()
+        for s, o in zip(singles, outs):
+            got = o.download()
+            assert np.array_equal(got.view(np.uint32), s.view(np.uint32)), rep
+    assert b.pairs > 0
+    b.close()
+
+
+def test_dropin_matches_oracle_gpu_semantics(gpu, oracle):
+    """azimuth_compress_ with files: DEM by fd, .geo read-modify-write by fd,
+    against the oracle's restatement of the reference GPU entry."""
+    src = synth.fbm_surface((256, 256), hmax=1200.0, seed=8)
+    sc = synth.make_burst(600, 5000, 200, 110, dem="fbm", dem_source=src, ntargets=3, seed=8,
+                          aperture=256)
+    g = sc.geom
+    data = synth.host_samples(sc)
+    full = np.concatenate([sc.dem[:3], sc.dem, sc.dem[-2:]]).astype(np.int16)
+    fl = g.firstlat - 3 * g.deltalat
+    latlons = np.array([fl + (g.nlines + 2.5) * g.deltalat, fl + 3.5 * g.deltalat,
+                        g.firstlon + 0.5 * g.deltalon,
+                        g.firstlon + (g.demwidth - 0.5) * g.deltalon])
+    rng = np.random.default_rng(3)
+    geo0 = (rng.standard_normal((full.shape[0], g.demwidth)) * 1e-3).astype(np.complex64)
+    res = {}
+    with tempfile.TemporaryDirectory() as td:
+        for who in ("gpu", "oracle"):
+            demf, geof = os.path.join(td, who + ".dem"), os.path.join(td, who + ".geo")
+            full.tofile(demf)
+            geo0.tofile(geof)
+            fdd, fdo = os.open(demf, os.O_RDWR), os.open(geof, os.O_RDWR)
+            if who == "gpu":
+                _call_dropin(data, sc, full.shape[0], fdo, fdd, fl, latlons)
+                assert T.lib().tdbp_dropin_status() == 0, T.lib().tdbp_last_error()
+            else:
+                oracle.azimuth_compress(data, sc.satloc, g.L, g.S, g.demwidth, full.shape[0],
+                                        fdo, fdd, g.deltalat, g.deltalon, fl, g.firstlon,
+                                        latlons, g.timeorbit, g.xx, g.vv, g.rngstart,
+                                        g.rngend, g.tmid, g.xyz_mid, g.vel_mid, g.dmrg, g.wvl,
+                                        g.A, g.angc0, g.angc1, g.prf,
+                                        semantics=pyoracle.GPU_SEMANTICS)
+            os.close(fdd)
+            os.close(fdo)
+            res[who] = np.fromfile(geof, np.complex64).reshape(full.shape[0], g.demwidth)
+    a, b = res["gpu"], res["oracle"]
+    changed = b != geo0
+    assert changed.sum() > 1000
+    np.testing.assert_array_equal(a != geo0, changed)
+    ph, amp = errors(a, b, changed)
+    assert ph <= 1e-6 and amp <= 1e-6
+    np.testing.assert_array_equal(a[~changed], geo0[~changed])
+
+
+def _call_dropin(data, sc, demlength, fdout, fddem, firstlat, latlons):
+    import ctypes as C
+
+    g = sc.geom
+    sat = np.zeros((100000, 3))
+    sat[:g.L] = sc.satloc
+    tt = np.zeros(1000)
+    tt[:len(g.timeorbit)] = g.timeorbit
+    xx = np.zeros((1000, 3))
+    xx[:len(g.timeorbit)] = g.xx
+    vv = np.zeros((1000, 3))
+    vv[:len(g.timeorbit)] = g.vv
+    ll = np.ascontiguousarray(latlons, np.float64)
+    xm = np.ascontiguousarray(g.xyz_mid, np.float64)
+    vm = np.ascontiguousarray(g.vel_mid, np.float64)
+    tp = np.zeros(100000)
+    b = np.ascontiguousarray(data, np.complex64)
+    i = lambda v: C.byref(C.c_int(int(v)))  # noqa: E731
+    d = lambda v: C.byref(C.c_double(float(v)))  # noqa: E731
+    dp = lambda a: a.ctypes.data_as(C.POINTER(C.c_double))  # noqa: E731
+    T.lib().azimuth_compress_(
+        b.ctypes.data, dp(sat), i(g.L), i(g.S), i(g.demwidth), i(demlength), i(fdout),
+        i(fddem), d(g.deltalat), d(g.deltalon), d(firstlat), d(g.firstlon), dp(ll), dp(tt),
+        dp(xx), dp(vv), i(len(g.timeorbit)), d(g.rngstart), d(g.rngend), d(0), d(0),
+        d(g.tmid), dp(xm), dp(vm), dp(tp), d(1 / g.prf), d(g.dmrg), d(g.wvl), i(g.A), i(0),
+        d(g.angc0), d(g.angc1), d(g.prf))
+
+
+def test_pack_roundtrip(gpu):
+    rng = np.random.default_rng(2)
+    L, S = 37, 1001
+    x = ((rng.standard_normal((L, S)) + 1j * rng.standard_normal((L, S))) *
+         np.exp(rng.uniform(-8, 8, (L, 1)))).astype(np.complex64)
+    x[5] = 0
+    d = T.DeviceArray.from_host(x)
+    for dt, eps in ((T.CF16, 2 ** -11), (T.CBF16, 2 ** -8)):
+        p, s = T.pack(d, L, S, dt)
+        raw = p.download()
+        sc = s.download()
+        if dt == T.CF16:
+            re = (raw & 0xFFFF).astype(np.uint16).view(np.float16).astype(np.float32)
+            im = (raw >> 16).astype(np.uint16).view(np.float16).astype(np.float32)
+        else:
+            re = ((raw & 0xFFFF) << 16).astype(np.uint32).view(np.float32)
+            im = (raw & 0xFFFF0000).astype(np.uint32).view(np.float32)
+        y = (re + 1j * im) * sc[:, None]
+        pmax = np.maximum(np.abs(x.real), np.abs(x.imag)).max(1, keepdims=True)
+        err = np.maximum(np.abs(y.real - x.real), np.abs(y.imag - x.imag))
+        assert np.all(err <= eps * pmax + 1e-30)
+        assert sc[5] == 1.0 and np.all(y[5] == 0)
