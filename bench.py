@@ -1,0 +1,269 @@
+"""Benchmark: batched mixed-precision TDBP of a 27-burst IW scene vs the plain
+FP64 kernel, on one MI355X.
+
+Workload (config 4): 3 subswaths x 9 bursts of uneven size, pulses
+U[1350,1550], samples U[18000,25000], output tiles U[1000,4500] x
+U[1200,1600] on an fBm DEM (seed 1).  Range-compressed inputs are complex64,
+synthesised on the GPU (band-limited circular Gaussian clutter + point
+targets) and packed to the storage type on the device; they are resident in
+HBM before timing starts.
+
+One step = one batched mixed-precision pass over all 27 bursts (one graph
+replay: queue reset, aperture fits, one persistent launch).  The plain FP64
+kernel is timed on the same bursts, one launch per burst as the drop-in runs
+it.  value = pixel-pulse pairs / step time, in Gpixel*pulse/s.
+
+Prints ONE JSON line.  --gpus N > 1 is rejected: this engine runs on one GPU.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "Gpixel·pulse/s backprojected, batched mixed vs plain FP64, ≤1 mrad RMS phase"
+FLOP_PER_PAIR = 30.0          # algorithmic work per pixel-pulse pair
+PEAK_FP32_TFLOPS = 157.3      # MI355X vector FP32
+PEAK_FP64_TFLOPS = 78.6       # MI355X vector FP64
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def errors(g, o):
+    g = g.astype(np.complex128).ravel()
+    o = o.astype(np.complex128).ravel()
+    m = np.abs(o) > 0
+    g, o = g[m], o[m]
+    w = np.abs(o) ** 2
+    ph = float(np.sqrt(np.sum(w * np.angle(g * np.conj(o)) ** 2) / np.sum(w)))
+    amp = float(np.sqrt(np.sum((np.abs(g) - np.abs(o)) ** 2) / np.sum(w)))
+    return ph, amp
+
+
+def cpu_baseline(sc, d_data, budget_s=15.0):
+    """The reference CPU pixelint (oracle/_ref, built from the reference's own
+    sources) on a bounded row sample of burst 0, all host threads; falls back
+    to the oracle restatement when the reference build is absent."""
+    from oracle import pyoracle
+
+    g = sc.geom
+    data = d_data.download()
+    orc = pyoracle.Oracle()
+    kind = "reference" if pyoracle.ref_available() else "port"
+    ref = pyoracle.RefCPU() if kind == "reference" else None
+    xyz, xyzfit = orc.setxyz(g, sc.dem)
+    coef = orc.fitcoef(g, xyzfit)
+    azoff = orc.setazoff(g, coef)
+    rows = np.nonzero((azoff > 0).any(1))[0]
+    mid = rows[len(rows) // 2]
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or os.cpu_count()
+
+    def run(r0, r1):
+        sl = slice(r0, r1)
+        t = time.perf_counter()
+        if ref is not None:
+            ref.pixelint(g, data, sc.satloc, xyz, azoff, rows=sl)
+        else:
+            orc.pixelint(g, data, sc.satloc, xyz, azoff, semantics=pyoracle.CPU_SEMANTICS,
+                         rows=sl)
+        dt = time.perf_counter() - t
+        a = azoff[sl]
+        s = np.floor(a - g.A // 2 + 0.5)      # C round() for positive values
+        e = np.minimum(np.floor(a + g.A // 2 + 0.5), g.L - 1)
+        pairs = int(np.sum(np.where(a > 0, e - np.maximum(s, 0), 0)))
+        return pairs, dt
+
+    pairs, dt = run(mid, mid + 2)                       # calibration
+    rate = pairs / max(dt, 1e-6)
+    nrows = int(max(2, min(len(rows), budget_s * rate / max(pairs / 2, 1))))
+    r0 = max(0, mid - nrows // 2)
+    pairs, dt = run(r0, r0 + nrows)
+    return {"value": pairs / dt / 1e9, "unit": "Gpixel·pulse/s", "cores": threads,
+            "kind": kind,
+            "sample": f"pixelint of burst 0 ({g.L}x{g.S}), window rows {r0}..{r0 + nrows - 1} "
+                      f"x {g.demwidth} px, {pairs} pairs in {dt:.1f} s, gcc -O2 -fopenmp, "
+                      f"OMP threads {threads}"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--storage", default="cf16", choices=["cf16", "cbf16", "cf32"])
+    ap.add_argument("--bursts", type=int, default=27)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--plain-passes", type=int, default=2)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for {args.gpus}); each burst is an "
+              f"independent unit and multi-GPU sharding is not part of this build",
+              file=sys.stderr)
+        sys.exit(2)
+
+    from srg_amd import synth
+    from srg_amd import tdbp as T
+
+    T.init(0)
+    t0 = time.time()
+    nsub = 3 if args.bursts >= 3 else 1
+    scenes = synth.config4(seed=1, nsub=nsub, nburst=max(1, args.bursts // nsub))
+    log(f"config 4 geometry: {len(scenes)} bursts in {time.time() - t0:.1f} s")
+    dts = {"cf32": T.CF32, "cf16": T.CF16, "cbf16": T.CBF16}
+    stor = dts[args.storage]
+
+    dev = []
+    for sc in scenes:
+        g = sc.geom
+        d = {"sc": sc,
+             "data": T.DeviceArray((g.L, g.S), np.complex64),
+             "sat": T.DeviceArray.from_host(sc.satloc),
+             "dem": T.DeviceArray.from_host(sc.dem),
+             "coef": T.DeviceArray((g.nlines, 3), np.float64),
+             "out": T.DeviceArray((g.nlines, g.demwidth), np.complex64),
+             "ref": T.DeviceArray((g.nlines, g.demwidth), np.complex64)}
+        synth.device_samples(sc, d["data"], d["sat"])
+        d["packed"] = {T.CF32: (d["data"], None)}
+        for dt in (T.CF16, T.CBF16):
+            d["packed"][dt] = T.pack(d["data"], g.L, g.S, dt)
+        T.fit_aperture(g, d["dem"], d["coef"])
+        dev.append(d)
+    T.sync()
+    log(f"bursts synthesised and packed on the GPU, {time.time() - t0:.1f} s")
+
+    def smp(d, dt):
+        data, scale = d["packed"][dt]
+        return T.samples(data, dt, scale)
+
+    # Autotune the mixed kernel's tile shape on the largest burst.
+    big = max(dev, key=lambda d: d["sc"].geom.nlines * d["sc"].geom.demwidth)
+    best, tune_ms = T.autotune(big["sc"].geom, smp(big, stor), big["sat"], big["dem"],
+                               big["out"], T.MIXED, reps=2)
+    log(f"autotune: variant {best} {T.variant_info(best)}; "
+        f"ms per variant {[round(x, 2) for x in tune_ms]}")
+
+    ev0, ev1 = T.Event(), T.Event()
+    stream = T.Stream()
+
+    # Plain FP64: one launch per burst (the drop-in's way), judge outputs.
+    def plain_pass():
+        for d in dev:
+            T.backproject(d["sc"].geom, smp(d, T.CF32), d["sat"], d["dem"], d["ref"],
+                          T.PLAIN_FP64, d_coef=d["coef"], stream=stream)
+
+    plain_pass()
+    stream.sync()
+    ev0.record(stream)
+    for _ in range(args.plain_passes):
+        plain_pass()
+    ev1.record(stream)
+    stream.sync()
+    plain_ms = ev0.elapsed_ms(ev1) / args.plain_passes
+
+    # Batched mixed precision, one graph replay per step.
+    batches = {}
+
+    def make_batch(dt):
+        b = T.Batch(T.MIXED, -1)
+        for d in dev:
+            b.add(d["sc"].geom, smp(d, dt), d["sat"], d["dem"], d["out"])
+        return b
+
+    batch = make_batch(stor)
+    batch.run(stream)
+    stream.sync()
+    pairs = batch.pairs
+    for _ in range(args.warmup):
+        batch.run(stream)
+    stream.sync()
+    ev0.record(stream)
+    for _ in range(args.steps):
+        batch.run(stream)
+    ev1.record(stream)
+    stream.sync()
+    mixed_ms = ev0.elapsed_ms(ev1) / args.steps
+    mixed_gpps = pairs / mixed_ms / 1e6
+    plain_gpps = pairs / plain_ms / 1e6
+    log(f"pairs {pairs}: mixed {args.storage} {mixed_ms:.2f} ms/step ({mixed_gpps:.1f} GP/s), "
+        f"plain FP64 {plain_ms:.2f} ms ({plain_gpps:.1f} GP/s)")
+
+    # Accuracy of the timed mixed outputs against the FP64 kernel (the judge).
+    num = {"w": 0.0, "ph": 0.0, "amp": 0.0}
+    for d in dev:
+        g_ = d["out"].download().astype(np.complex128).ravel()
+        o_ = d["ref"].download().astype(np.complex128).ravel()
+        m = np.abs(o_) > 0
+        g_, o_ = g_[m], o_[m]
+        w = np.abs(o_) ** 2
+        num["w"] += float(np.sum(w))
+        num["ph"] += float(np.sum(w * np.angle(g_ * np.conj(o_)) ** 2))
+        num["amp"] += float(np.sum((np.abs(g_) - np.abs(o_)) ** 2))
+    ph = float(np.sqrt(num["ph"] / num["w"]))
+    amp = float(np.sqrt(num["amp"] / num["w"]))
+    log(f"mixed {args.storage} vs FP64 kernel: phase {ph * 1e3:.4f} mrad, amp {amp:.2e}")
+
+    # The other storage types, batched the same way (reported, not the value).
+    others = {}
+    for name, dt in dts.items():
+        if dt == stor:
+            continue
+        b = make_batch(dt)
+        b.run(stream)
+        stream.sync()
+        ev0.record(stream)
+        for _ in range(max(3, args.steps // 4)):
+            b.run(stream)
+        ev1.record(stream)
+        stream.sync()
+        ms = ev0.elapsed_ms(ev1) / max(3, args.steps // 4)
+        e_ph, e_amp = 0.0, 0.0
+        d = dev[0]
+        e_ph, e_amp = errors(d["out"].download(), d["ref"].download())
+        others[name] = {"gpps": round(pairs / ms / 1e6, 1), "ms_per_step": round(ms, 3),
+                        "burst0_phase_mrad": round(e_ph * 1e3, 4), "burst0_amp_rel": e_amp}
+        b.close()
+        log(f"mixed {name}: {ms:.2f} ms ({pairs / ms / 1e6:.1f} GP/s), burst 0 phase "
+            f"{e_ph * 1e3:.4f} mrad amp {e_amp:.2e}")
+
+    achieved = pairs * FLOP_PER_PAIR / (mixed_ms * 1e-3) / 1e12
+    roof = {"bound": "valu", "achieved": round(achieved, 3), "peak": PEAK_FP32_TFLOPS,
+            "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_TFLOPS, 4), "traffic": None,
+            "kernel": "k_batch<MIXED> (persistent)", "flop_per_pair": FLOP_PER_PAIR,
+            "pairs_per_launch": pairs,
+            "plain_fp64_frac": round(pairs * FLOP_PER_PAIR / (plain_ms * 1e-3) / 1e12 /
+                                     PEAK_FP64_TFLOPS, 4)}
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(dev[0]["sc"], dev[0]["data"])
+        log(f"cpu baseline: {cpu}")
+
+    line = {
+        "metric": METRIC, "value": round(mixed_gpps, 2), "unit": "Gpixel·pulse/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(mixed_ms, 3), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32 (f64 per-tile reference)",
+        "data": "synthetic (GPU-generated band-limited clutter + point targets, fBm DEM)",
+        "config": {"workload": "config 4: IW scene, 3 subswaths x 9 uneven bursts, batched",
+                   "bursts": len(dev), "pairs_per_step": pairs, "storage": args.storage,
+                   "variant": batch.variant, "tile": T.variant_info(batch.variant)[:2]},
+        "plain_fp64": {"value": round(plain_gpps, 2), "ms_per_pass": round(plain_ms, 3),
+                       "launch": "one per burst"},
+        "speedup_vs_plain_fp64": round(plain_ms / mixed_ms, 3),
+        "error_vs_fp64": {"phase_mrad_rms": round(ph * 1e3, 5), "amp_rel_rms": amp},
+        "other_storage": others,
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+    batch.close()
+
+
+if __name__ == "__main__":
+    main()

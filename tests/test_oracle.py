@@ -305,3 +305,27 @@ def test_point_target_focuses_with_its_phase(oracle):
         # resolution) the neighbours are much dimmer than the focus
         nb = np.abs(out[r, [c - 2, c - 1, c + 1, c + 2]])
         assert nb.max() < 0.5 * abs(out[r, c])
+
+
+def test_gpu_semantics_branch_vs_cpu_branch(oracle):
+    """The reference GPU kernel cannot be compiled here (no CUDA), so its
+    semantics are pinned relative to the CPU twin, which is bit-exact with
+    the reference: same window and gate, the sum scaled by 1/(azend-azstart),
+    the lerp weights and products in FP32 (sentinel/azimuth_compress.cu:69,
+    :78-79).  A missing or wrong-sided 1/N, or an FP64 lerp, fails this."""
+    sc, b = _small()
+    g = sc.geom
+    xyz, fit = oracle.setxyz(g, sc.dem)
+    az = oracle.setazoff(g, oracle.fitcoef(g, fit))
+    oc, ac, _ = oracle.pixelint(g, b, sc.satloc, xyz, az, semantics=pyoracle.CPU_SEMANTICS)
+    og, ag, _ = oracle.pixelint(g, b, sc.satloc, xyz, az, semantics=pyoracle.GPU_SEMANTICS)
+    m = az > 0
+    s = np.round(az[m] - g.A // 2)
+    e = np.minimum(np.floor(az[m] + g.A // 2 + 0.5), g.L - 1)
+    n = e - np.maximum(s, 0)
+    # accumulators differ only by the FP32 vs FP64 lerp (~1e-7 relative)
+    d = np.abs(ag[m] - ac[m]) / np.maximum(np.abs(ac[m]), 1e-30)
+    assert np.median(d) < 1e-6 and 0 < np.median(d)
+    want = (ac[m].astype(np.complex64) * (1.0 / n).astype(np.float32)).astype(np.complex64)
+    np.testing.assert_allclose(og[m], want, rtol=1e-5, atol=1e-7 * np.abs(want).max())
+    assert np.all(og[~m] == 0)
