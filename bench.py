@@ -47,6 +47,21 @@ def errors(g, o):
     return ph, amp
 
 
+def measured_traffic(storage, variant, pairs):
+    """HBM bytes per launch from the committed PMC summary (FETCH_SIZE and
+    WRITE_SIZE runs, calibrated by kernels of known size) when it was measured
+    on this exact workload; else None."""
+    import re
+    path = os.path.join(ROOT, "profiles", "k_batch_mixed_c4_pmc.md")
+    if not os.path.exists(path):
+        return None
+    m = re.search(r"traffic_bytes_per_launch=(\d+) storage=(\w+) variant=(\d+) pairs=(\d+)",
+                  open(path).read())
+    if not m or m.group(2) != storage or int(m.group(3)) != variant or int(m.group(4)) != pairs:
+        return None
+    return int(m.group(1))
+
+
 def cpu_baseline(sc, d_data, budget_s=15.0):
     """The reference CPU pixelint (oracle/_ref, built from the reference's own
     sources) on a bounded row sample of burst 0, all host threads; falls back
@@ -101,6 +116,10 @@ def main():
     ap.add_argument("--bursts", type=int, default=27)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--plain-passes", type=int, default=2)
+    ap.add_argument("--profile", action="store_true",
+                    help="counter-collection run: a 1 GiB fill and a 1 GiB copy kernel "
+                         "(calibration of WRITE_SIZE / FETCH_SIZE) then two eager launches "
+                         "of the batched mixed kernel; prints the pairs per launch")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); each burst is an "
@@ -141,6 +160,25 @@ def main():
     def smp(d, dt):
         data, scale = d["packed"][dt]
         return T.samples(data, dt, scale)
+
+    if args.profile:
+        os.environ["TDBP_GRAPH"] = "0"   # eager dispatches, one per kernel
+        nbytes = 1 << 30
+        a = T.DeviceArray((nbytes,), np.uint8)
+        b = T.DeviceArray((nbytes,), np.uint8)
+        a.zero()
+        T.check(T.lib().tdbp_dev_copy(b.ptr, a.ptr, nbytes, None))
+        T.sync()
+        batch = T.Batch(T.MIXED, int(os.environ.get("TDBP_VARIANT", "5")))
+        for d in dev:
+            batch.add(d["sc"].geom, smp(d, stor), d["sat"], d["dem"], d["out"])
+        batch.run()
+        batch.run()
+        T.sync()
+        print(json.dumps({"profile": True, "copy_bytes": nbytes, "pairs_per_launch": batch.pairs,
+                          "variant": batch.variant, "storage": args.storage}), flush=True)
+        batch.close()
+        return
 
     # Autotune the mixed kernel's tile shape on the largest burst.
     big = max(dev, key=lambda d: d["sc"].geom.nlines * d["sc"].geom.demwidth)
@@ -234,7 +272,8 @@ def main():
 
     achieved = pairs * FLOP_PER_PAIR / (mixed_ms * 1e-3) / 1e12
     roof = {"bound": "valu", "achieved": round(achieved, 3), "peak": PEAK_FP32_TFLOPS,
-            "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_TFLOPS, 4), "traffic": None,
+            "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_TFLOPS, 4),
+            "traffic": measured_traffic(args.storage, batch.variant, pairs),
             "kernel": "k_batch<MIXED> (persistent)", "flop_per_pair": FLOP_PER_PAIR,
             "pairs_per_launch": pairs,
             "plain_fp64_frac": round(pairs * FLOP_PER_PAIR / (plain_ms * 1e-3) / 1e12 /

@@ -113,6 +113,8 @@ int tdbp_dev_free(void *d_ptr);
 int tdbp_memcpy_htod(void *d_dst, const void *src, size_t bytes, void *stream);
 int tdbp_memcpy_dtoh(void *dst, const void *d_src, size_t bytes, void *stream);
 int tdbp_memcpy_dtod(void *d_dst, const void *d_src, size_t bytes, void *stream);
+/* Device-to-device copy with a copy kernel (graph-capture safe). */
+int tdbp_dev_copy(void *d_dst, const void *d_src, size_t bytes, void *stream);
 /* Zero device memory with a fill kernel (graph-capture safe). */
 int tdbp_dev_zero(void *d_ptr, size_t bytes, void *stream);
 int tdbp_stream_create(void **stream);

@@ -312,6 +312,10 @@ int tdbp_memcpy_dtod(void* d, const void* src, size_t n, void* s) {
       "memcpy D2D");
 }
 
+int tdbp_dev_copy(void* d, const void* src, size_t n, void* s) {
+  return hip_check(launch_copy(d, src, n, (hipStream_t)s), "copy");
+}
+
 int tdbp_dev_zero(void* d, size_t n, void* s) {
   return hip_check(launch_zero(d, n, (hipStream_t)s), "zero");
 }

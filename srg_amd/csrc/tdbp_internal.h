@@ -69,6 +69,7 @@ hipError_t launch_batch(const BurstDev* d_bursts, const int2* d_items,
 hipError_t launch_batch_fit(const BurstDev* d_bursts, int nbursts,
                             int max_lines, hipStream_t s);
 hipError_t launch_zero(void* p, size_t bytes, hipStream_t s);
+hipError_t launch_copy(void* d, const void* src, size_t bytes, hipStream_t s);
 hipError_t launch_pack(const float2* in, int L, int S, int dtype, void* out,
                        float* scale, hipStream_t s);
 hipError_t launch_synth_clutter(float2* d, int L, int S, uint64_t seed,

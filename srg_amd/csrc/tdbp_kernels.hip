@@ -28,6 +28,11 @@
 #include "tdbp_geom.h"
 #include "tdbp_internal.h"
 
+// Unroll of the test-free core pulse loop.
+#ifndef TDBP_CORE_UNROLL
+#define TDBP_CORE_UNROLL 1
+#endif
+
 namespace tdbp {
 
 const Variant kVariants[kNumVariants] = {
@@ -40,6 +45,14 @@ typedef const double __attribute__((address_space(4)))* const_dptr;
 __device__ __forceinline__ int xcd_tile(int b, int nb) {
   const int q = nb >> 3, r = nb & 7, x = b & 7, j = b >> 3;
   return x * q + (x < r ? x : r) + j;
+}
+
+// A value known to be equal on every lane, moved to scalar registers.
+__device__ __forceinline__ double uniform(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readfirstlane((int)(b & 0xffffffffll));
+  const int hi = __builtin_amdgcn_readfirstlane((int)(b >> 32));
+  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
 }
 
 __device__ __forceinline__ int wave_min(int v) {
@@ -159,43 +172,11 @@ __device__ __forceinline__ void plain_tile(const BurstDev& b, int tile) {
 
 // ------------------------------------------------------- mixed precision
 
-// Sample pair loads, returned as FP32 (re0, im0, re1, im1) times nothing:
-// the per-pulse scale is applied to the phasor.
-template <int DT>
-__device__ __forceinline__ float4 load_pair(const void* rowp, int i) {
-  if constexpr (DT == TDBP_CF32) {
-    const float2* r = (const float2*)rowp;
-    const float2 a = r[i], c = r[i + 1];
-    return make_float4(a.x, a.y, c.x, c.y);
-  } else if constexpr (DT == TDBP_CF16) {
-    const uint32_t* r = (const uint32_t*)rowp;
-    uint2 w;
-    __builtin_memcpy(&w, r + i, 8);
-    typedef _Float16 h2 __attribute__((ext_vector_type(2)));
-    h2 u0, u1;
-    __builtin_memcpy(&u0, &w.x, 4);
-    __builtin_memcpy(&u1, &w.y, 4);
-    return make_float4((float)u0.x, (float)u0.y, (float)u1.x, (float)u1.y);
-  } else {
-    const uint32_t* r = (const uint32_t*)rowp;
-    uint2 w;
-    __builtin_memcpy(&w, r + i, 8);
-    return make_float4(__uint_as_float(w.x << 16),
-                       __uint_as_float(w.x & 0xffff0000u),
-                       __uint_as_float(w.y << 16),
-                       __uint_as_float(w.y & 0xffff0000u));
-  }
-}
-
-template <int DT>
-__device__ __forceinline__ int elem_bytes() {
-  return DT == TDBP_CF32 ? 8 : 4;
-}
-
 // Per-pulse record in LDS (3 x float4, written once per workgroup stage):
-//   r0 = (du.x, du.y, du.z, c0)      du = u_p - u_ref, c0 = frac(K R0) cycles
-//   r1 = (f0, i0 bits, 1/R0, 1/(2 R0))   (R0 - rngstart)/dmrg = i0 + f0
-//   r2 = (lim, scale, -, -)          gate: rho <= lim  <=>  r <= r_hi
+//   r0 = (du.x, du.y, du.z, c0)   du = u_p - u_ref, c0 = frac(2 R0/lambda)
+//   r1 = (f0, i0*elem bits, 1/(2 R0), scale)  (R0 - rngstart)/dmrg = i0 + f0
+//   r2 = (safe, lim, i0 bits, -)   lim: rho <= lim <=> r <= r_hi; safe != 0:
+//        every pixel of the workgroup tile is inside the range gate
 struct PulseRec {
   float4 r0, r1, r2;
 };
@@ -206,7 +187,93 @@ constexpr int kStage = kBlock;  // pulses per stage = one per thread
 struct TileShared {
   double x0[3];      // reference point
   int key, wlo, whi;
+  unsigned dmax;     // bits of max |x - x0| over live pixels (float, >= 0)
 };
+
+// Global-address-space view of the sample rows: keeps the gather a
+// global_load (scalar base + 32-bit lane offset) even when the row pointer
+// comes from the burst table in memory.
+typedef const char __attribute__((address_space(1)))* grow_t;
+typedef uint32_t U32x2 __attribute__((ext_vector_type(2), aligned(4)));
+typedef float F32x4 __attribute__((ext_vector_type(4), aligned(8)));
+template <class T>
+__device__ __forceinline__ T gload(grow_t p) {
+  return *(const __attribute__((address_space(1))) T*)p;
+}
+
+// Sample pair (i, i+1) at byte offset `off` from a uniform row base, lerped
+// with weight fr and scaled by the per-pulse scale sc.  The weights carry the
+// scale: sc*(1-fr), sc*fr.  For cf16, fma(b1, w1, fma(b0, w0, +0)) keeps the
+// f16 operands inside v_fma_mix_f32 (an fma with +0 cannot become a multiply).
+template <int DT>
+__device__ __forceinline__ float2 lerp_pair(grow_t rowb, unsigned off, float fr,
+                                            float sc) {
+  if constexpr (DT == TDBP_CF32) {
+    const F32x4 v = gload<F32x4>(rowb + off);
+    return make_float2(fmaf(fr, v.z - v.x, v.x), fmaf(fr, v.w - v.y, v.y));
+  } else if constexpr (DT == TDBP_CF16) {
+    const U32x2 w = gload<U32x2>(rowb + off);
+    typedef _Float16 h2 __attribute__((ext_vector_type(2)));
+    const h2 u0 = __builtin_bit_cast(h2, (uint32_t)w.x);
+    const h2 u1 = __builtin_bit_cast(h2, (uint32_t)w.y);
+    const float w1 = fr * sc, w0 = sc - w1;
+    return make_float2(fmaf((float)u1.x, w1, fmaf((float)u0.x, w0, 0.0f)),
+                       fmaf((float)u1.y, w1, fmaf((float)u0.y, w0, 0.0f)));
+  } else {
+    const U32x2 w = gload<U32x2>(rowb + off);
+    const float w1 = fr * sc, w0 = sc - w1;
+    return make_float2(
+        fmaf(__uint_as_float(w.y << 16), w1, __uint_as_float(w.x << 16) * w0),
+        fmaf(__uint_as_float(w.y & 0xffff0000u), w1,
+             __uint_as_float(w.x & 0xffff0000u) * w0));
+  }
+}
+
+template <int DT>
+__device__ __forceinline__ int elem_bytes() {
+  return DT == TDBP_CF32 ? 8 : 4;
+}
+
+// Per-lane constants of the mixed pulse loop.
+struct MixedPixel {
+  float dx, dy, dz;   // x - x0
+  float bb;           // |x - x0|^2
+  float aref;         // u_ref . (x - x0)
+  float arefd;        // aref / dmrg
+  float w;            // 1 - aref/R0 + (aref/R0)^2: 1/(R0+a) ~ w/R0
+};
+
+// One pixel-pulse pair; CHECK selects the per-lane window and gate tests.
+// rowb is the uniform start of pulse row q; the lane's byte offset is
+// (floor(rho) + i0) * elem, non-negative whenever the gate holds.
+template <int DT, bool CHECK>
+__device__ __forceinline__ void mixed_pair(const MixedPixel& m, const PulseRec& r,
+                                           grow_t rowb, float Kf, float invd,
+                                           bool inwin, unsigned smax, float& sr,
+                                           float& si) {
+  const float da = fmaf(r.r0.x, m.dx, fmaf(r.r0.y, m.dy, r.r0.z * m.dz));
+  const float a = m.aref + da;
+  const float t = fmaf(-a, a, m.bb);
+  const float del = t * r.r1.z * m.w;
+  const float sd = da + del;                      // R - R0 - aref
+  const float rho = fmaf(sd, invd, r.r1.x + m.arefd);
+  int il;  // floor(rho) as an integer in one instruction
+  asm("v_cvt_flr_i32_f32 %0, %1" : "=v"(il) : "v"(rho));
+  if constexpr (CHECK) {
+    const int idx = il + __float_as_int(r.r2.z);
+    if (!(inwin && (unsigned)idx <= smax && rho <= r.r2.y)) return;
+  }
+  const float fr = __builtin_amdgcn_fractf(rho);
+  const unsigned off =
+      (unsigned)il * (unsigned)elem_bytes<DT>() + (unsigned)__float_as_int(r.r1.y);
+  const float2 c = lerp_pair<DT>(rowb, off, fr, r.r1.w);
+  const float cyc = fmaf(Kf, sd, r.r0.w);
+  const float sn = __builtin_amdgcn_sinf(cyc), cs = __builtin_amdgcn_cosf(cyc);
+  sr = fmaf(c.x, cs, sr);
+  sr = fmaf(-c.y, sn, sr);
+  si = fmaf(c.x, sn, si);
+  si = fmaf(c.y, cs, si);
+}
 
 template <int TR, bool STACK, int DT>
 __device__ __forceinline__ void mixed_tile(const BurstDev& b, int tile,
@@ -229,6 +296,7 @@ __device__ __forceinline__ void mixed_tile(const BurstDev& b, int tile,
     sh->key = 0x7fffffff;
     sh->wlo = 0x7fffffff;
     sh->whi = -1;
+    sh->dmax = 0u;
   }
   __syncthreads();
   if (live) {
@@ -249,90 +317,128 @@ __device__ __forceinline__ void mixed_tile(const BurstDev& b, int tile,
     sh->x0[2] = x[2];
   }
   __syncthreads();
-  const double x0[3] = {sh->x0[0], sh->x0[1], sh->x0[2]};
+  const double x0[3] = {uniform(sh->x0[0]), uniform(sh->x0[1]), uniform(sh->x0[2])};
 
-  // Unit look vector at the window's central pulse.
+  // Unit look vector and range at the window's central pulse.
   const int pc = (wlo + whi - 1) >> 1;
-  double uref[3];
+  double uref[3], R0ref;
   {
     const double* s = b.satloc + 3 * pc;
-    double u[3] = {x0[0] - s[0], x0[1] - s[1], x0[2] - s[2]};
-    const double inv = 1.0 / sqrt(u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
-    uref[0] = u[0] * inv;
-    uref[1] = u[1] * inv;
-    uref[2] = u[2] * inv;
+    const double u[3] = {x0[0] - s[0], x0[1] - s[1], x0[2] - s[2]};
+    R0ref = uniform(sqrt(u[0] * u[0] + u[1] * u[1] + u[2] * u[2]));
+    const double inv = 1.0 / R0ref;
+    uref[0] = uniform(u[0] * inv);
+    uref[1] = uniform(u[1] * inv);
+    uref[2] = uniform(u[2] * inv);
   }
 
-  // Per-pixel constants.
-  const double d64[3] = {x[0] - x0[0], x[1] - x0[1], x[2] - x0[2]};
-  const float dx = (float)d64[0], dy = (float)d64[1], dz = (float)d64[2];
-  const float bb = (float)(d64[0] * d64[0] + d64[1] * d64[1] + d64[2] * d64[2]);
+  // Per-pixel constants (lanes without a pixel sit on x0).
+  const double d64[3] = {live ? x[0] - x0[0] : 0., live ? x[1] - x0[1] : 0.,
+                         live ? x[2] - x0[2] : 0.};
+  const double bb64 = d64[0] * d64[0] + d64[1] * d64[1] + d64[2] * d64[2];
   const double aref64 = uref[0] * d64[0] + uref[1] * d64[1] + uref[2] * d64[2];
-  const float aref = (float)aref64;
+  const double q = aref64 / R0ref;
+  MixedPixel m;
+  m.dx = (float)d64[0];
+  m.dy = (float)d64[1];
+  m.dz = (float)d64[2];
+  m.bb = (float)bb64;
+  m.aref = (float)aref64;
+  m.arefd = (float)(aref64 * b.inv_dmrg);
+  m.w = (float)(1.0 - q + q * q);
   const double cp64 = aref64 * b.two_over_wvl;
   const float cpix = (float)(cp64 - rint(cp64));
+  if (live) atomicMax(&sh->dmax, __float_as_uint((float)sqrt(bb64)));
   const float Kf = (float)b.two_over_wvl;
   const float invd = (float)b.inv_dmrg;
 
+  // Wave window [qlo, qhi) and core [qc0, qc1) where every live lane of the
+  // wave is inside its own window.
   const int qlo = wave_min(live ? p.azs : 0x7fffffff);
   const int qhi = wave_max(live ? p.aze : -1);
+  int qc0 = wave_max(live ? p.azs : -1);
+  int qc1 = wave_min(live ? p.aze : 0x7fffffff);
+  if (qc1 < qc0) qc1 = qc0;
+  const bool wave_live = qhi > qlo;
   const unsigned n = (unsigned)(p.aze - p.azs);
-  const char* base = (const char*)b.data;
+  const grow_t base = (grow_t)b.data;
   const size_t rowbytes = (size_t)b.S * elem_bytes<DT>();
   const unsigned smax = (unsigned)(b.S - 2);
+  __syncthreads();
+  // Conservative range extent of the tile about x0: R - R0 in
+  // [-dmax, dmax + dmax^2/(2(R0-dmax))], plus one sample of margin.
+  const double dmax =
+      (double)__uint_as_float((unsigned)__builtin_amdgcn_readfirstlane((int)sh->dmax));
 
   float acc_r = 0.f, acc_i = 0.f, cmp_r = 0.f, cmp_i = 0.f;
   for (int p0 = wlo; p0 < whi; p0 += kStage) {
     // Stage prologue (FP64): one pulse per thread.
-    const int q = p0 + tid;
-    if (q < whi) {
-      const double* s = b.satloc + 3 * q;
+    const int qs = p0 + tid;
+    bool safe_all = true;
+    if (qs < whi) {
+      const double* s = b.satloc + 3 * qs;
       const double u0 = x0[0] - s[0], u1 = x0[1] - s[1], u2 = x0[2] - s[2];
       const double R0 = sqrt(u0 * u0 + u1 * u1 + u2 * u2);
       const double inv = 1.0 / R0;
       const double c = R0 * b.two_over_wvl;
       const double rb = (R0 - b.rngstart) * b.inv_dmrg;
       const double i0 = floor(rb);
+      const double rlo = rb - dmax * b.inv_dmrg - 1.0;
+      const double rhi = rb + (dmax + dmax * dmax / (2.0 * (R0 - dmax))) * b.inv_dmrg + 1.0;
+      const bool safe = rlo >= 1.0 && rhi <= b.r_hi - 1.0;
       PulseRec r;
       r.r0 = make_float4((float)(u0 * inv - uref[0]), (float)(u1 * inv - uref[1]),
                          (float)(u2 * inv - uref[2]), (float)(c - rint(c)));
-      r.r1 = make_float4((float)(rb - i0), __int_as_float((int)i0), (float)inv,
-                         (float)(0.5 * inv));
-      const float sc = (DT == TDBP_CF32 || b.scale == nullptr) ? 1.f : b.scale[q];
-      r.r2 = make_float4((float)(b.r_hi - i0), sc, 0.f, 0.f);
+      const float sc = (DT == TDBP_CF32 || b.scale == nullptr) ? 1.f : b.scale[qs];
+      r.r1 = make_float4((float)(rb - i0), __int_as_float((int)i0 * elem_bytes<DT>()),
+                         (float)(0.5 * inv), sc);
+      r.r2 = make_float4(safe ? 1.f : 0.f, (float)(b.r_hi - i0), __int_as_float((int)i0),
+                         0.f);
       rec[tid] = r;
+      safe_all = safe;
     }
-    __syncthreads();
-    const int qa = max(p0, qlo), qb = min(p0 + kStage, qhi);
+    // Barrier that also tells whether every pulse of the stage is safe.
+    const int stage_safe = __syncthreads_and(safe_all ? 1 : 0);
     float sr = 0.f, si = 0.f;
-    for (int qq = qa; qq < qb; ++qq) {
-      const PulseRec r = rec[qq - p0];
-      const float da = fmaf(r.r0.x, dx, fmaf(r.r0.y, dy, r.r0.z * dz));
-      const float a = aref + da;
-      const float t = fmaf(-a, a, bb);
-      const float w = fmaf(-a, r.r1.z, 1.0f);
-      const float del = t * r.r1.w * w;
-      const float rho = fmaf(a + del, invd, r.r1.x);
-      const float fl = floorf(rho);
-      const int idx = (int)fl + __float_as_int(r.r1.y);
-      if ((unsigned)(qq - p.azs) < n && (unsigned)idx <= smax &&
-          rho <= r.r2.x) {
-        const float fr = rho - fl;
-        const float4 v = load_pair<DT>(base + (size_t)qq * rowbytes, idx);
-        const float cr = fmaf(fr, v.z - v.x, v.x);
-        const float ci = fmaf(fr, v.w - v.y, v.y);
-        const float cyc = fmaf(Kf, da + del, r.r0.w + cpix);
-        const float ph = __builtin_amdgcn_fractf(cyc);
-        float sn = __builtin_amdgcn_sinf(ph), cs = __builtin_amdgcn_cosf(ph);
-        if constexpr (DT != TDBP_CF32) {
-          sn *= r.r2.y;
-          cs *= r.r2.y;
+    // A wave without a live pixel skips the pulse loops (its window bounds
+    // are sentinels and must not enter index arithmetic).
+    if (wave_live) {
+      const int qa = max(p0, qlo), qb = min(p0 + kStage, qhi);
+      const int ea = min(qb, qc0);                      // [qa, ea): checked
+      const int ca = max(qa, qc0), cb = min(qb, qc1);   // core
+      const int eb = max(qa, qc1);                      // [eb, qb): checked
+      for (int qq = qa; qq < ea; ++qq)
+        mixed_pair<DT, true>(m, rec[qq - p0], base + (size_t)qq * rowbytes, Kf, invd,
+                             (unsigned)(qq - p.azs) < n, smax, sr, si);
+      if (stage_safe) {
+        // Every pulse of the stage keeps the whole tile inside the gate and
+        // every live lane of the wave is inside its window: no tests.
+        int qq = ca;
+#if TDBP_CORE_UNROLL > 1
+        for (; qq < cb - 1; qq += 2) {
+          mixed_pair<DT, false>(m, rec[qq - p0], base + (size_t)qq * rowbytes, Kf, invd,
+                                true, smax, sr, si);
+          mixed_pair<DT, false>(m, rec[qq + 1 - p0], base + (size_t)(qq + 1) * rowbytes,
+                                Kf, invd, true, smax, sr, si);
         }
-        sr = fmaf(cr, cs, sr);
-        sr = fmaf(-ci, sn, sr);
-        si = fmaf(cr, sn, si);
-        si = fmaf(ci, cs, si);
+#endif
+        for (; qq < cb; ++qq)
+          mixed_pair<DT, false>(m, rec[qq - p0], base + (size_t)qq * rowbytes, Kf, invd,
+                                true, smax, sr, si);
+      } else {
+        for (int qq = ca; qq < cb; ++qq) {
+          const PulseRec r = rec[qq - p0];
+          const grow_t rowb = base + (size_t)qq * rowbytes;
+          if (__builtin_amdgcn_readfirstlane(__float_as_int(r.r2.x)) != 0)
+            mixed_pair<DT, false>(m, r, rowb, Kf, invd, true, smax, sr, si);
+          else
+            mixed_pair<DT, true>(m, r, rowb, Kf, invd, (unsigned)(qq - p.azs) < n, smax,
+                                 sr, si);
+        }
       }
+      for (int qq = eb; qq < qb; ++qq)
+        mixed_pair<DT, true>(m, rec[qq - p0], base + (size_t)qq * rowbytes, Kf, invd,
+                             (unsigned)(qq - p.azs) < n, smax, sr, si);
     }
     // Kahan-compensated join of the stage sums.
     {
@@ -345,7 +451,12 @@ __device__ __forceinline__ void mixed_tile(const BurstDev& b, int tile,
     }
     __syncthreads();
   }
-  store_out(b, p, acc_r - cmp_r, acc_i - cmp_i);
+  // The per-pixel phase offset frac(2/lambda u_ref.d) is common to all
+  // pulses: apply it once.
+  float ps, pcs;
+  sincospif(2.0f * cpix, &ps, &pcs);
+  const float fr_ = acc_r - cmp_r, fi_ = acc_i - cmp_i;
+  store_out(b, p, fr_ * pcs - fi_ * ps, fr_ * ps + fi_ * pcs);
 }
 
 // ----------------------------------------------------------- kernels
@@ -356,8 +467,16 @@ __global__ __launch_bounds__(kBlock) void k_plain(BurstDev b) {
   plain_tile<TR, STACK>(b, xcd_tile(blockIdx.x, nb));
 }
 
+// Occupancy target of the mixed kernels: 8 waves per SIMD, the hardware
+// maximum (<= 64 VGPRs); the pulse loop is a dependent gather, so resident
+// waves hide its latency (measured: 7 -> 8 waves is +9 % on config 4).
+#ifndef TDBP_MIXED_WAVES
+#define TDBP_MIXED_WAVES 8
+#endif
+#define TDBP_MIXED_OCC __attribute__((amdgpu_waves_per_eu(TDBP_MIXED_WAVES)))
+
 template <int TR, bool STACK, int DT>
-__global__ __launch_bounds__(kBlock) void k_mixed(BurstDev b) {
+__global__ __launch_bounds__(kBlock) TDBP_MIXED_OCC void k_mixed(BurstDev b) {
   __shared__ PulseRec rec[kStage];
   __shared__ TileShared sh;
   const int nb = b.tiles_r * b.tiles_c;
@@ -367,23 +486,21 @@ __global__ __launch_bounds__(kBlock) void k_mixed(BurstDev b) {
 // Persistent batch kernel: workgroup on XCD x = blockIdx % 8 drains queue x,
 // then steals from the others.  Queue k holds items [qbase[k], qbase[k+1]).
 template <int PREC, int TR, bool STACK, int DT>
-__global__ __launch_bounds__(kBlock) void k_batch(const BurstDev* __restrict__ bursts,
-                                                  const int2* __restrict__ items,
-                                                  const int* __restrict__ qbase,
-                                                  int* counters) {
-  __shared__ PulseRec rec[PREC == TDBP_MIXED ? kStage : 1];
-  __shared__ TileShared sh;
-  __shared__ int item_s;
+__device__ __forceinline__ void batch_body(const BurstDev* __restrict__ bursts,
+                                           const int2* __restrict__ items,
+                                           const int* __restrict__ qbase,
+                                           int* counters, PulseRec* rec,
+                                           TileShared* sh, int* item_s) {
   const int home = blockIdx.x & 7;
   int qi = 0;
   while (qi < 8) {
     const int qk = (home + qi) & 7;
     if (threadIdx.x == 0) {
       const int k = atomicAdd(&counters[qk], 1);
-      item_s = (k < qbase[qk + 1] - qbase[qk]) ? qbase[qk] + k : -1;
+      *item_s = (k < qbase[qk + 1] - qbase[qk]) ? qbase[qk] + k : -1;
     }
     __syncthreads();
-    const int it = item_s;
+    const int it = *item_s;
     __syncthreads();
     if (it < 0) {
       ++qi;
@@ -392,12 +509,31 @@ __global__ __launch_bounds__(kBlock) void k_batch(const BurstDev* __restrict__ b
     const int2 w = items[it];
     const BurstDev& bd = bursts[w.x];
     if constexpr (PREC == TDBP_MIXED) {
-      mixed_tile<TR, STACK, DT>(bd, w.y, rec, &sh);
+      mixed_tile<TR, STACK, DT>(bd, w.y, rec, sh);
     } else {
       plain_tile<TR, STACK>(bd, w.y);
     }
     __syncthreads();
   }
+}
+
+template <int TR, bool STACK, int DT>
+__global__ __launch_bounds__(kBlock) TDBP_MIXED_OCC void k_batch_mixed(
+    const BurstDev* __restrict__ bursts, const int2* __restrict__ items,
+    const int* __restrict__ qbase, int* counters) {
+  __shared__ PulseRec rec[kStage];
+  __shared__ TileShared sh;
+  __shared__ int item_s;
+  batch_body<TDBP_MIXED, TR, STACK, DT>(bursts, items, qbase, counters, rec, &sh, &item_s);
+}
+
+template <int TR, bool STACK>
+__global__ __launch_bounds__(kBlock) void k_batch_plain(
+    const BurstDev* __restrict__ bursts, const int2* __restrict__ items,
+    const int* __restrict__ qbase, int* counters) {
+  __shared__ int item_s;
+  batch_body<TDBP_PLAIN_FP64, TR, STACK, TDBP_CF32>(bursts, items, qbase, counters,
+                                                     nullptr, nullptr, &item_s);
 }
 
 // One thread per window line: three zero-Doppler solves, the aperture
@@ -534,6 +670,20 @@ __global__ void k_zero16(uint4* p, size_t n16, uint8_t* tail, int ntail) {
        i += stride)
     p[i] = make_uint4(0u, 0u, 0u, 0u);
   if (blockIdx.x == 0 && (int)threadIdx.x < ntail) tail[threadIdx.x] = 0;
+}
+
+__global__ void k_copy16(uint4* __restrict__ d, const uint4* __restrict__ s, size_t n16,
+                         uint8_t* dt, const uint8_t* st, int ntail) {
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n16; i += stride)
+    d[i] = s[i];
+  if (blockIdx.x == 0 && (int)threadIdx.x < ntail) dt[threadIdx.x] = st[threadIdx.x];
+}
+
+__global__ void k_copy1(uint8_t* __restrict__ d, const uint8_t* __restrict__ s, size_t n) {
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+    d[i] = s[i];
 }
 
 __global__ void k_zero1(uint8_t* p, size_t n) {
@@ -682,17 +832,26 @@ hipError_t mixed_dt(const BurstDev& b, hipStream_t s) {
 template <int PREC, int TR, bool STACK, int DT>
 hipError_t batch_launch(const BurstDev* d, const int2* it, const int* qb,
                         int* ctr, int grid, hipStream_t s) {
-  hipLaunchKernelGGL((k_batch<PREC, TR, STACK, DT>), dim3(grid), dim3(kBlock),
-                     0, s, d, it, qb, ctr);
+  if constexpr (PREC == TDBP_MIXED)
+    hipLaunchKernelGGL((k_batch_mixed<TR, STACK, DT>), dim3(grid), dim3(kBlock), 0, s,
+                       d, it, qb, ctr);
+  else
+    hipLaunchKernelGGL((k_batch_plain<TR, STACK>), dim3(grid), dim3(kBlock), 0, s, d,
+                       it, qb, ctr);
   return hipGetLastError();
 }
 
 template <int PREC, int TR, bool STACK, int DT>
 int occupancy() {
   int n = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
-          &n, k_batch<PREC, TR, STACK, DT>, kBlock, 0) != hipSuccess)
-    return 1;
+  hipError_t e;
+  if constexpr (PREC == TDBP_MIXED)
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_batch_mixed<TR, STACK, DT>,
+                                                     kBlock, 0);
+  else
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_batch_plain<TR, STACK>,
+                                                     kBlock, 0);
+  if (e != hipSuccess) return 1;
   return n > 0 ? n : 1;
 }
 
@@ -822,6 +981,24 @@ hipError_t launch_zero(void* p, size_t bytes, hipStream_t s) {
   g = g == 0 ? 1 : (g < 65535 ? g : 65535);
   hipLaunchKernelGGL(k_zero16, dim3((unsigned)g), dim3(kBlock), 0, s,
                      (uint4*)p, n16, (uint8_t*)p + n16 * 16, ntail);
+  return hipGetLastError();
+}
+
+hipError_t launch_copy(void* d, const void* src, size_t bytes, hipStream_t s) {
+  if (bytes == 0) return hipSuccess;
+  if ((uintptr_t)d % 16 != 0 || (uintptr_t)src % 16 != 0) {
+    size_t g = blocks_for(bytes, kBlock);
+    hipLaunchKernelGGL(k_copy1, dim3((unsigned)(g < 65535 ? g : 65535)), dim3(kBlock), 0,
+                       s, (uint8_t*)d, (const uint8_t*)src, bytes);
+    return hipGetLastError();
+  }
+  const size_t n16 = bytes / 16;
+  const int ntail = (int)(bytes - n16 * 16);
+  size_t g = blocks_for(n16, kBlock);
+  g = g == 0 ? 1 : (g < 65535 ? g : 65535);
+  hipLaunchKernelGGL(k_copy16, dim3((unsigned)g), dim3(kBlock), 0, s, (uint4*)d,
+                     (const uint4*)src, n16, (uint8_t*)d + n16 * 16,
+                     (const uint8_t*)src + n16 * 16, ntail);
   return hipGetLastError();
 }
 

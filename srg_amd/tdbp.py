@@ -14,7 +14,7 @@ from typing import Optional
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libsrg_tdbp.so")
+LIB_PATH = os.environ.get("TDBP_LIB") or os.path.join(_HERE, "libsrg_tdbp.so")
 
 # tdbp_status
 OK, ERR_ARG, ERR_HIP, ERR_NODEV, ERR_IO, ERR_NOMEM, ERR_LIMIT = range(7)
@@ -92,6 +92,7 @@ def _sig(lib):
         "tdbp_memcpy_dtoh": (C.c_int, [_VP, _VP, C.c_size_t, _VP]),
         "tdbp_memcpy_dtod": (C.c_int, [_VP, _VP, C.c_size_t, _VP]),
         "tdbp_dev_zero": (C.c_int, [_VP, C.c_size_t, _VP]),
+        "tdbp_dev_copy": (C.c_int, [_VP, _VP, C.c_size_t, _VP]),
         "tdbp_stream_create": (C.c_int, [C.POINTER(_VP)]),
         "tdbp_stream_destroy": (C.c_int, [_VP]),
         "tdbp_stream_sync": (C.c_int, [_VP]),
@@ -146,7 +147,7 @@ def exported_symbols():
     return [
         "tdbp_last_error", "tdbp_device_count", "tdbp_init", "tdbp_dropin_status",
         "tdbp_dev_alloc", "tdbp_dev_free", "tdbp_memcpy_htod", "tdbp_memcpy_dtoh",
-        "tdbp_memcpy_dtod", "tdbp_dev_zero", "tdbp_stream_create", "tdbp_stream_destroy",
+        "tdbp_memcpy_dtod", "tdbp_dev_zero", "tdbp_dev_copy", "tdbp_stream_create", "tdbp_stream_destroy",
         "tdbp_stream_sync", "tdbp_device_sync", "tdbp_event_create", "tdbp_event_destroy",
         "tdbp_event_record", "tdbp_event_elapsed_ms", "tdbp_mem_info", "tdbp_window",
         "tdbp_orbit_interp", "tdbp_setxyz", "tdbp_fit_aperture", "tdbp_setazoff",
