@@ -1,7 +1,7 @@
 """Benchmark: batched mixed-precision TDBP of a 27-burst IW scene vs the plain
 FP64 kernel, on one MI355X.
 
-Workload (config 4): 3 subswaths x 9 bursts of uneven size, pulses
+Workload (IW scene): 3 subswaths x 9 bursts of uneven size, pulses
 U[1350,1550], samples U[18000,25000], output tiles U[1000,4500] x
 U[1200,1600] on an fBm DEM (seed 1).  Range-compressed inputs are complex64,
 synthesised on the GPU (band-limited circular Gaussian clutter + point
@@ -52,7 +52,7 @@ def measured_traffic(storage, variant, pairs):
     WRITE_SIZE runs, calibrated by kernels of known size) when it was measured
     on this exact workload; else None."""
     import re
-    path = os.path.join(ROOT, "profiles", "k_batch_mixed_c4_pmc.md")
+    path = os.path.join(ROOT, "profiles", "k_batch_mixed_iw_scene_pmc.md")
     if not os.path.exists(path):
         return None
     m = re.search(r"traffic_bytes_per_launch=(\d+) storage=(\w+) variant=(\d+) pairs=(\d+)",
@@ -108,13 +108,15 @@ def cpu_baseline(sc, d_data, budget_s=15.0):
 
 
 def single_burst(args, synth, T):
-    """Configs 2 and 3: one burst, FP64 kernel vs mixed for each storage type,
-    errors against the FP64 kernel (config 2) or the FP64 oracle (config 3)."""
+    """One burst, FP64 kernel vs mixed for each storage type: the full IW burst
+    with errors against the FP64 kernel, or the small burst against the FP64
+    oracle."""
     from oracle import pyoracle
 
-    sc = synth.config2() if args.config == 2 else synth.config1()
+    full = args.workload == "burst"
+    sc = synth.iw_burst() if full else synth.small_burst()
     g = sc.geom
-    if args.config == 2:
+    if full:
         d_data = T.DeviceArray((g.L, g.S), np.complex64)
         d_sat = T.DeviceArray.from_host(sc.satloc)
         synth.device_samples(sc, d_data, d_sat)
@@ -142,15 +144,15 @@ def single_burst(args, synth, T):
         return ev0.elapsed_ms(ev1) / reps, d_out.download()
 
     plain_ms, plain = timed(T.samples(d_data), T.PLAIN_FP64, max(2, args.steps // 4))
-    if args.config == 3:
+    if not full:
         ref = pyoracle.Oracle().backproject(g, d_data.download(), sc.satloc, sc.dem)[0]
-        judge = "FP64 oracle (reference GPU semantics)"
+        against = "FP64 oracle (reference GPU semantics)"
     else:
         ref = plain
-        judge = "FP64 kernel"
+        against = "FP64 kernel"
     m = ref != 0
     res = {"plain_fp64": {"gpps": round(pairs / plain_ms / 1e6, 2), "ms": round(plain_ms, 4),
-                          "vs_judge": dict(zip(("phase_mrad", "amp_rel"),
+                          "vs_reference": dict(zip(("phase_mrad", "amp_rel"),
                                                (lambda e: (round(e[0] * 1e3, 6), e[1]))(
                                                    errors(plain[m], ref[m]))))}}
     for name, dt in (("cf32", T.CF32), ("cf16", T.CF16), ("cbf16", T.CBF16)):
@@ -171,11 +173,114 @@ def single_burst(args, synth, T):
             "steps": args.steps, "warmup": 1, "ms_per_step": best["ms"],
             "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
             "dtype": "f32 (f64 per-tile reference)", "data": "synthetic",
-            "config": {"workload": f"config {args.config}: " + (
-                "one IW burst 1500 x 21000 -> 4000 x 1500 px, fBm DEM" if args.config == 2 else
+            "config": {"workload": (
+                "one IW burst 1500 x 21000 -> 4000 x 1500 px, fBm DEM" if full else
                 "512 x 4096 burst -> 256 x 256 px, flat DEM, A=256"),
-                "pairs": pairs, "judge": judge},
+                "pairs": pairs, "errors_against": against},
             "results": res}
+    print(json.dumps(line), flush=True)
+
+
+def stack_bench(args, synth, T):
+    """Time-series stack: 30 acquisitions x 9 uneven bursts onto one common
+    13500 x 4000 grid, cf16 mixed precision, admitted into HBM-sized batches,
+    each batch one persistent launch plus the device .geo merges; per
+    acquisition the merged grid is compared with the FP64 kernel's."""
+    from srg_amd import stack as ST
+
+    t0 = time.time()
+    stk = synth.ts_stack(nacq=args.acqs)
+    grid = stk.grid
+    nacq = len(stk.bursts)
+    log(f"stack geometry: {nacq} acquisitions, {sum(len(r) for r in stk.bursts)} bursts, "
+        f"{time.time() - t0:.1f} s")
+    d_dem = T.DeviceArray.from_host(grid.dem)
+    bursts = [ST.StackBurst(a, sc.geom, sc.satloc,
+                            d_dem.rows(sc.geom.firstline, sc.geom.nlines), payload=sc)
+              for a, k, sc in stk.items()]
+    g_mixed = [T.DeviceArray((grid.lines, grid.W), np.complex64) for _ in range(nacq)]
+    g_fp64 = [T.DeviceArray((grid.lines, grid.W), np.complex64) for _ in range(nacq)]
+    for x in g_mixed + g_fp64:
+        x.zero()
+    maxls = max(b.geom.L * b.geom.S for b in bursts)
+    maxnl = max(b.geom.nlines for b in bursts)
+    staging = T.DeviceArray((maxls,), np.complex64)
+    out64 = T.DeviceArray((maxnl, grid.W), np.complex64)
+    coef = T.DeviceArray((maxnl, 3), np.float64)
+    stream = T.Stream()
+    e0, e1 = T.Event(), T.Event()
+    plain = {"ms": 0.0}
+
+    def loader(i, b, r):
+        # Synthesise cf32 samples, run the FP64 kernel on them into the FP64
+        # grids (timed per launch), then pack the resident cf16 copy.
+        g = b.geom
+        stg = staging.rows(0, g.L * g.S)
+        synth.device_samples(b.payload, stg, r.sat, stream=stream)
+        o = out64.rows(0, g.nlines)
+        c = coef.rows(0, g.nlines)
+        T.fit_aperture(g, b.dem, c, stream=stream)
+        e0.record(stream)
+        T.backproject(g, T.samples(stg), r.sat, b.dem, o, T.PLAIN_FP64, d_coef=c,
+                      stream=stream)
+        e1.record(stream)
+        T.merge_geo(g_fp64[b.acq], g.firstline, o, stream=stream)
+        T.pack_into(stg, g.L, g.S, T.CF16, r.buffers[0], r.buffers[1], stream=stream)
+        stream.sync()
+        plain["ms"] += e0.elapsed_ms(e1)
+
+    tuned = {}
+
+    def tune(idx, res):
+        if tuned:
+            return
+        k = max(range(len(idx)), key=lambda j: bursts[idx[j]].geom.nlines)
+        g = bursts[idx[k]].geom
+        r = res[k]
+        best, ms = T.autotune(g, r.samples, r.sat, r.dem, r.out, T.MIXED, reps=2)
+        tuned["variant"] = best
+        log(f"autotune: variant {best} {T.variant_info(best)}")
+
+    budget = int(args.hbm_budget * 2**30) if args.hbm_budget else None
+    run = ST.run_stack(bursts, g_mixed, loader, precision=T.MIXED, dtype=T.CF16,
+                       budget=budget, repeats=args.warmup + args.steps, on_resident=tune)
+    pairs = sum(run.pairs)
+    per_batch = [float(np.mean(ms[args.warmup:])) for ms in run.run_ms]
+    step_ms = sum(per_batch)
+    gpps = pairs / step_ms / 1e6
+    plain_gpps = pairs / plain["ms"] / 1e6
+    log(f"{len(run.batches)} batch(es), pairs {pairs}: {step_ms:.1f} ms/step "
+        f"({gpps:.1f} GP/s); FP64 kernel per burst {plain['ms']:.1f} ms ({plain_gpps:.1f} GP/s)")
+    per_acq = []
+    for a in range(nacq):
+        e = T.compare(g_mixed[a], g_fp64[a])
+        per_acq.append(e)
+    ph = [e["phase_rms"] * 1e3 for e in per_acq]
+    amp = [e["amp_rms"] for e in per_acq]
+    log(f"per-acquisition phase vs FP64 (mrad): max {max(ph):.4f} mean {np.mean(ph):.4f}; "
+        f"amp max {max(amp):.2e}")
+    line = {
+        "metric": METRIC, "value": round(gpps, 2), "unit": "Gpixel·pulse/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(step_ms, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32 (f64 per-tile reference)",
+        "data": "synthetic (GPU-generated band-limited clutter + point targets, fBm DEM)",
+        "config": {"workload": f"time-series stack: {nacq} acquisitions x 9 uneven bursts -> "
+                               f"one {grid.lines} x {grid.W} grid, cf16, .geo merge on device",
+                   "bursts": len(bursts), "pairs_per_step": pairs,
+                   "batches": len(run.batches),
+                   "batch_sizes": [len(b) for b in run.batches],
+                   "budget_bytes": budget, "variant": tuned.get("variant")},
+        "plain_fp64": {"value": round(plain_gpps, 2), "ms_per_pass": round(plain["ms"], 3),
+                       "launch": "one per burst"},
+        "speedup_vs_plain_fp64": round(plain["ms"] / step_ms, 3),
+        "error_vs_fp64_per_acquisition": {
+            "phase_mrad_rms": [round(x, 5) for x in ph],
+            "amp_rel_rms": [float(f"{x:.3e}") for x in amp],
+            "phase_mrad_max": round(max(ph), 5), "amp_rel_max": max(amp),
+            "extra_nonzero_pixels": sum(e["extra_nonzero"] for e in per_acq)},
+        "baselines_m": [round(float(x), 2) for x in np.linalg.norm(stk.baselines, axis=1)],
+    }
     print(json.dumps(line), flush=True)
 
 
@@ -188,10 +293,15 @@ def main():
     ap.add_argument("--bursts", type=int, default=27)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--plain-passes", type=int, default=2)
-    ap.add_argument("--config", type=int, default=4, choices=[2, 3, 4],
-                    help="4: batched 27-burst IW scene (the headline); 2: one full IW "
-                         "burst, plain vs mixed; 3: config 1 (512 x 4096, 256 x 256) per "
-                         "storage type against the FP64 oracle")
+    ap.add_argument("--workload", default="scene",
+                    choices=["scene", "burst", "small", "stack"],
+                    help="scene: batched 27-burst IW scene (the headline); burst: one full "
+                         "IW burst, plain vs mixed per storage type; small: the 512 x 4096 "
+                         "-> 256 x 256 burst per storage type against the FP64 oracle; "
+                         "stack: 30 acquisitions x 9 bursts onto one 13500 x 4000 grid")
+    ap.add_argument("--acqs", type=int, default=30, help="stack: acquisitions")
+    ap.add_argument("--hbm-budget", type=float, default=0.0,
+                    help="stack: admission budget in GiB (default: free HBM less a reserve)")
     ap.add_argument("--profile", action="store_true",
                     help="counter-collection run: a 1 GiB fill and a 1 GiB copy kernel "
                          "(calibration of WRITE_SIZE / FETCH_SIZE) then two eager launches "
@@ -207,12 +317,14 @@ def main():
     from srg_amd import tdbp as T
 
     T.init(0)
-    if args.config in (2, 3):
+    if args.workload in ("burst", "small"):
         return single_burst(args, synth, T)
+    if args.workload == "stack":
+        return stack_bench(args, synth, T)
     t0 = time.time()
     nsub = 3 if args.bursts >= 3 else 1
-    scenes = synth.config4(seed=1, nsub=nsub, nburst=max(1, args.bursts // nsub))
-    log(f"config 4 geometry: {len(scenes)} bursts in {time.time() - t0:.1f} s")
+    scenes = synth.iw_scene(seed=1, nsub=nsub, nburst=max(1, args.bursts // nsub))
+    log(f"IW scene geometry: {len(scenes)} bursts in {time.time() - t0:.1f} s")
     dts = {"cf32": T.CF32, "cf16": T.CF16, "cbf16": T.CBF16}
     stor = dts[args.storage]
 
@@ -268,7 +380,8 @@ def main():
     ev0, ev1 = T.Event(), T.Event()
     stream = T.Stream()
 
-    # Plain FP64: one launch per burst (the drop-in's way), judge outputs.
+    # Plain FP64: one launch per burst (the drop-in's way); its outputs are the
+    # accuracy reference.
     def plain_pass():
         for d in dev:
             T.backproject(d["sc"].geom, smp(d, T.CF32), d["sat"], d["dem"], d["ref"],
@@ -310,7 +423,7 @@ def main():
     log(f"pairs {pairs}: mixed {args.storage} {mixed_ms:.2f} ms/step ({mixed_gpps:.1f} GP/s), "
         f"plain FP64 {plain_ms:.2f} ms ({plain_gpps:.1f} GP/s)")
 
-    # Accuracy of the timed mixed outputs against the FP64 kernel (the judge).
+    # Accuracy of the timed mixed outputs against the FP64 kernel.
     num = {"w": 0.0, "ph": 0.0, "amp": 0.0}
     for d in dev:
         g_ = d["out"].download().astype(np.complex128).ravel()
@@ -367,7 +480,9 @@ def main():
         "ms_per_step": round(mixed_ms, 3), "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f32 (f64 per-tile reference)",
         "data": "synthetic (GPU-generated band-limited clutter + point targets, fBm DEM)",
-        "config": {"workload": "config 4: IW scene, 3 subswaths x 9 uneven bursts, batched",
+        "config": {"workload": "IW scene: 3 subswaths x 9 uneven bursts (pulses U[1350,1550], "
+                               "samples U[18000,25000], windows U[1000,4500] x U[1200,1600] "
+                               "px), batched",
                    "bursts": len(dev), "pairs_per_step": pairs, "storage": args.storage,
                    "variant": batch.variant, "tile": T.variant_info(batch.variant)[:2]},
         "plain_fp64": {"value": round(plain_gpps, 2), "ms_per_pass": round(plain_ms, 3),

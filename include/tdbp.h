@@ -232,6 +232,14 @@ tdbp_batch *tdbp_batch_create(int precision, int variant);
  * or -tdbp_status. */
 int tdbp_batch_add(tdbp_batch *b, const tdbp_geom *g, const tdbp_samples *s,
                    const double *d_satloc, const int16_t *d_dem, void *d_out);
+/* Merge burst `burst`'s output (its [nlines][W] rows) into rows
+ * [firstline, firstline+nlines) of a device grid [grid_lines][grid_width]
+ * (grid_width must equal the burst's demwidth) after every run, in burst
+ * order, with the .geo rule of sentinel/azimuth_compress.cu:527-531 (pixel
+ * replaced where |Re| > 1e-18).  Set before the first run.  Replaces the
+ * per-burst .geo read-modify-write of sentinel/azimuth_compress.cu:513-536. */
+int tdbp_batch_set_merge(tdbp_batch *b, int burst, void *d_grid, int grid_lines,
+                         int grid_width, int firstline);
 /* Run (first call builds the queue and captures the graph). */
 int tdbp_batch_run(tdbp_batch *b, void *stream);
 /* Variant the batch runs (resolved at the first run). */
@@ -241,6 +249,22 @@ int64_t tdbp_batch_pairs(const tdbp_batch *b);
 /* Device memory held by the batch itself (queue, descriptors, coef). */
 size_t tdbp_batch_device_bytes(const tdbp_batch *b);
 void tdbp_batch_destroy(tdbp_batch *b);
+
+/* The .geo merge on the device: burst output d_out [nlines][W] into rows
+ * [firstline, firstline+nlines) of d_grid [grid_lines][W], replacing a pixel
+ * where |Re(out)| > 1e-18 (sentinel/azimuth_compress.cu:527-531). */
+int tdbp_merge_geo(void *d_grid, int W, int grid_lines, int firstline, int nlines,
+                   const void *d_out, void *stream);
+
+/* Error sums of a cf32 image against a cf32 reference, n pixels, over the
+ * pixels where the reference is nonzero: sums[0] = sum |ref|^2 dphi^2,
+ * sums[1] = sum |ref|^2, sums[2] = sum (|img| - |ref|)^2, sums[3] = count,
+ * sums[4] = pixels where ref is zero and img is not.  RMS phase =
+ * sqrt(sums[0]/sums[1]); RMS relative amplitude = sqrt(sums[2]/sums[1]).
+ * Synchronises the stream.  No reference counterpart: this is the per-config
+ * precision report of the mixed path against the FP64 path. */
+int tdbp_compare(const void *d_img, const void *d_ref, size_t n, double *sums,
+                 void *stream);
 
 /* ------------------------------------------- synthetic data (bench/tests) */
 

@@ -63,6 +63,7 @@ class Oracle:
             C.c_double, C.c_double, C.c_double, _D, _D, _D, _D, C.c_int, C.c_double,
             C.c_double, C.c_double, _D, _D, C.c_double, C.c_double, C.c_int, C.c_double,
             C.c_double, C.c_double, C.c_int, C.c_int]
+        L.orc_merge_geo.argtypes = [_F, _F, C.c_long, C.c_int]
         self.lib = L
 
     # -- orbit
@@ -151,6 +152,17 @@ class Oracle:
         self.lib.orc_window(_p(ll, _D), firstlat, firstlon, deltalat, deltalon, W, demlength,
                             *[C.byref(x) for x in o])
         return tuple(x.value for x in o)
+
+    def merge_geo(self, grid, out, firstline, semantics=GPU_SEMANTICS):
+        """Merge a burst window out [nlines][W] into grid rows [firstline, ...)
+        in place (grid: C-contiguous complex64)."""
+        assert grid.dtype == np.complex64 and grid.flags.c_contiguous
+        out = _c(out, np.complex64)
+        nl, W = out.shape
+        assert grid.shape[1] == W and firstline + nl <= grid.shape[0]
+        win = grid[firstline:firstline + nl]
+        self.lib.orc_merge_geo(_p(win.view(np.float32), _F), _p(out.view(np.float32), _F),
+                               nl * W, semantics)
 
     def azimuth_compress(self, burst, satloc, L, S, W, demlength, fdout, fddem, dlat, dlon,
                          firstlat, firstlon, latlons, t, x, v, rngstart, rngend, tmid,

@@ -345,6 +345,22 @@ int64_t orc_backproject(const float *burst, const double *satloc,
   return pairs;
 }
 
+/* The .geo merge of one burst window: grid pixel k takes out[k] where the
+ * merge test on Re(out[k]) passes.  CPU twin: abs((int)re) > 1e-18, i.e. only
+ * |re| >= 1 (sentinel/azimuth_compress_cpu.c:387); GPU entry:
+ * fabs((double)re) > 1e-18 (sentinel/azimuth_compress.cu:527-531). */
+void orc_merge_geo(float *grid, const float *out, long npx, int semantics) {
+  for (long k = 0; k < npx; ++k) {
+    float re = out[2 * k];
+    int take = semantics == ORC_CPU_SEMANTICS ? (abs((int)re) > 1.e-18)
+                                              : (fabs((double)re) > 1.e-18);
+    if (take) {
+      grid[2 * k] = out[2 * k];
+      grid[2 * k + 1] = out[2 * k + 1];
+    }
+  }
+}
+
 /* Entry point with the reference file semantics: DEM rows read from fddem at
  * firstline*W*2, .geo rows read and written back at firstline*W*8, a pixel
  * overwritten where the merge test passes.  Restates
@@ -384,15 +400,7 @@ void orc_azimuth_compress(const float *burst, const double *satloc,
   off_t off = (off_t)firstline * demwidth * 8;
   ssize_t got = pread(fdout, old, npx * 8, off);
   (void)got;
-  for (long k = 0; k < npx; ++k) {
-    float re = out[2 * k];
-    int take = semantics == ORC_CPU_SEMANTICS ? (abs((int)re) > 1.e-18)
-                                              : (fabs((double)re) > 1.e-18);
-    if (take) {
-      old[2 * k] = out[2 * k];
-      old[2 * k + 1] = out[2 * k + 1];
-    }
-  }
+  orc_merge_geo(old, out, npx, semantics);
   if (pwrite(fdout, old, npx * 8, off) != (ssize_t)(npx * 8))
     fprintf(stderr, "orc: geo write error\n");
   free(dem);

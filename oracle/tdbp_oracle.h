@@ -96,6 +96,10 @@ int64_t orc_backproject(const float *burst, const double *satloc,
                         int samplesPerBurst, int nfit, int semantics,
                         float *out, double *acc, int nthreads);
 
+/* .geo merge of a burst window (npx cf32 pixels) into the grid rows it
+ * covers: CPU twin abs((int)re) > 1e-18, GPU entry fabs(re) > 1e-18. */
+void orc_merge_geo(float *grid, const float *out, long npx, int semantics);
+
 /* Full entry point with file I/O (DEM read by fd, .geo read-modify-write by
  * fd).  semantics selects the CPU twin (int-abs merge, no 1/N) or the GPU
  * entry (fabs merge, 1/N).  nfit_last: 0 reproduces the reference loop bound

@@ -312,6 +312,49 @@ int tdbp_memcpy_dtod(void* d, const void* src, size_t n, void* s) {
       "memcpy D2D");
 }
 
+int tdbp_merge_geo(void* d_grid, int W, int grid_lines, int firstline, int nlines,
+                   const void* d_out, void* stream) {
+  if (!d_grid || !d_out || W < 1 || nlines < 1 || firstline < 0 ||
+      firstline + nlines > grid_lines)
+    return set_error(TDBP_ERR_ARG, "merge: rows [%d, %d) outside a %d-line grid", firstline,
+                     firstline + nlines, grid_lines);
+  int st = current_device_ready();
+  if (st) return st;
+  return hip_check(launch_merge((float2*)d_grid, (const float2*)d_out, (size_t)nlines * W,
+                                (size_t)firstline * W, (hipStream_t)stream),
+                   "merge launch");
+}
+
+int tdbp_compare(const void* d_img, const void* d_ref, size_t n, double* sums,
+                 void* stream) {
+  if (!d_img || !d_ref || !sums) return set_error(TDBP_ERR_ARG, "compare: null pointer");
+  int st = current_device_ready();
+  if (st) return st;
+  hipStream_t s = (hipStream_t)stream;
+  const int blocks = compare_blocks(n);
+  double* part = nullptr;
+  if ((st = hip_check(hipMallocAsync((void**)&part, (size_t)blocks * 5 * sizeof(double), s),
+                      "compare scratch")))
+    return st;
+  std::vector<double> hp((size_t)blocks * 5);
+  st = hip_check(launch_compare((const float2*)d_img, (const float2*)d_ref, n, part,
+                                blocks, s),
+                 "compare launch");
+  if (!st)
+    st = hip_check(hipMemcpyAsync(hp.data(), part, hp.size() * sizeof(double),
+                                  hipMemcpyDeviceToHost, s),
+                   "compare download");
+  const int st2 = hip_check(hipFreeAsync(part, s), "compare scratch free");
+  const int st3 = hip_check(hipStreamSynchronize(s), "compare sync");
+  if (st) return st;
+  if (st2) return st2;
+  if (st3) return st3;
+  for (int k = 0; k < 5; ++k) sums[k] = 0.0;
+  for (int b = 0; b < blocks; ++b)
+    for (int k = 0; k < 5; ++k) sums[k] += hp[(size_t)b * 5 + k];
+  return TDBP_OK;
+}
+
 int tdbp_dev_copy(void* d, const void* src, size_t n, void* s) {
   return hip_check(launch_copy(d, src, n, (hipStream_t)s), "copy");
 }

@@ -20,11 +20,17 @@
 
 using namespace tdbp;
 
+struct MergeTarget {
+  float2* grid = nullptr;   // nullptr: no merge for this burst
+  int grid_lines = 0, firstline = 0;
+};
+
 struct tdbp_batch {
   int precision = TDBP_MIXED;
   int variant = -1;
   int dtype = -1;
   std::vector<BurstDev> bursts;
+  std::vector<MergeTarget> merges;  // per burst, applied in burst order
   std::vector<void*> owned;  // device allocations (orbits, coefs, tables)
   BurstDev* d_bursts = nullptr;
   int2* d_items = nullptr;
@@ -56,10 +62,22 @@ int enqueue(tdbp_batch* b, hipStream_t s) {
                                   b->max_lines, s),
                  "batch fit launch");
   if (st) return st;
-  return hip_check(launch_batch(b->d_bursts, b->d_items, b->d_qbase,
-                                b->d_counters, b->precision, b->variant,
-                                b->dtype, b->grid, s),
-                   "batch launch");
+  st = hip_check(launch_batch(b->d_bursts, b->d_items, b->d_qbase, b->d_counters,
+                              b->precision, b->variant, b->dtype, b->grid, s),
+                 "batch launch");
+  if (st) return st;
+  // .geo merges in burst order: overlapping rows take the later burst, as the
+  // reference's sequential read-modify-write does.
+  for (size_t i = 0; i < b->bursts.size(); ++i) {
+    const MergeTarget& m = b->merges[i];
+    if (!m.grid) continue;
+    const BurstDev& bd = b->bursts[i];
+    st = hip_check(launch_merge(m.grid, bd.out, (size_t)bd.nlines * bd.W,
+                                (size_t)m.firstline * bd.W, s),
+                   "merge launch");
+    if (st) return st;
+  }
+  return TDBP_OK;
 }
 
 int build(tdbp_batch* b) {
@@ -210,7 +228,27 @@ int tdbp_batch_add(tdbp_batch* b, const tdbp_geom* g, const tdbp_samples* s,
   point_orbit(&bd, d_orb, g->numstatevec);
   bd.coef = d_coef;
   b->bursts.push_back(bd);
+  b->merges.push_back(MergeTarget());
   return (int)b->bursts.size() - 1;
+}
+
+int tdbp_batch_set_merge(tdbp_batch* b, int burst, void* d_grid, int grid_lines,
+                         int grid_width, int firstline) {
+  if (!b || burst < 0 || burst >= (int)b->bursts.size())
+    return set_error(TDBP_ERR_ARG, "merge target for unknown burst %d", burst);
+  if (b->built) return set_error(TDBP_ERR_ARG, "batch already built");
+  const BurstDev& bd = b->bursts[burst];
+  if (grid_width != bd.W)
+    return set_error(TDBP_ERR_ARG, "merge grid width %d != burst DEM width %d", grid_width,
+                     bd.W);
+  if (!d_grid || firstline < 0 || firstline + bd.nlines > grid_lines)
+    return set_error(TDBP_ERR_ARG, "merge rows [%d, %d) outside a %d-line grid", firstline,
+                     firstline + bd.nlines, grid_lines);
+  MergeTarget& m = b->merges[burst];
+  m.grid = (float2*)d_grid;
+  m.grid_lines = grid_lines;
+  m.firstline = firstline;
+  return TDBP_OK;
 }
 
 int tdbp_batch_run(tdbp_batch* b, void* stream) {

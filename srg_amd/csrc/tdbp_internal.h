@@ -70,6 +70,11 @@ hipError_t launch_batch_fit(const BurstDev* d_bursts, int nbursts,
                             int max_lines, hipStream_t s);
 hipError_t launch_zero(void* p, size_t bytes, hipStream_t s);
 hipError_t launch_copy(void* d, const void* src, size_t bytes, hipStream_t s);
+int compare_blocks(size_t n);
+hipError_t launch_compare(const float2* g, const float2* o, size_t n, double* part,
+                          int blocks, hipStream_t s);
+hipError_t launch_merge(float2* grid, const float2* out, size_t n, size_t grid_off,
+                        hipStream_t s);
 hipError_t launch_pack(const float2* in, int L, int S, int dtype, void* out,
                        float* scale, hipStream_t s);
 hipError_t launch_synth_clutter(float2* d, int L, int S, uint64_t seed,

@@ -9,7 +9,7 @@
 // range-sample gather is a per-lane memory access.
 //
 //  * k_plain : FP64 range, cycle-reduced FP64 phase, FP32 lerp, FP64
-//              accumulator -- the drop-in and the judge.
+//              accumulator -- the drop-in and the accuracy reference.
 //  * k_mixed : per workgroup, a FP64 reference point x0; per pulse (FP64,
 //              once per workgroup) range R0, unit look vector u, and the
 //              phase/range bases; per pixel-pulse only FP32:
@@ -120,7 +120,7 @@ __device__ __forceinline__ void store_out(const BurstDev& b, const PixelCtx& p,
   b.out[(size_t)p.row * b.W + p.col] = o;
 }
 
-// ------------------------------------------------------- FP64 pair (judge)
+// ------------------------------------- FP64 pair (drop-in, accuracy reference)
 
 // One pixel-pulse pair in FP64: range, gate, FP32 lerp of the two samples,
 // phase exp(+j 4 pi R / lambda) reduced in cycles before sincospi.
@@ -469,7 +469,7 @@ __global__ __launch_bounds__(kBlock) void k_plain(BurstDev b) {
 
 // Occupancy target of the mixed kernels: 8 waves per SIMD, the hardware
 // maximum (<= 64 VGPRs); the pulse loop is a dependent gather, so resident
-// waves hide its latency (measured: 7 -> 8 waves is +9 % on config 4).
+// waves hide its latency (measured: 7 -> 8 waves is +9 % on the 27-burst IW scene).
 #ifndef TDBP_MIXED_WAVES
 #define TDBP_MIXED_WAVES 8
 #endif
@@ -670,6 +670,53 @@ __global__ void k_zero16(uint4* p, size_t n16, uint8_t* tail, int ntail) {
        i += stride)
     p[i] = make_uint4(0u, 0u, 0u, 0u);
   if (blockIdx.x == 0 && (int)threadIdx.x < ntail) tail[threadIdx.x] = 0;
+}
+
+// .geo merge of one burst output into a grid (sentinel/azimuth_compress.cu:527-531):
+// a pixel is replaced where |Re(out)| > 1e-18 (the comparison in double).
+__global__ void k_merge(float2* __restrict__ grid, const float2* __restrict__ out,
+                        size_t n, size_t grid_off) {
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    const float2 v = out[i];
+    if (fabs((double)v.x) > 1.e-18) grid[grid_off + i] = v;
+  }
+}
+
+// Error sums of an image g against a reference o over the pixels where o is
+// nonzero: [0] sum |o|^2 arg(g conj o)^2, [1] sum |o|^2, [2] sum (|g|-|o|)^2,
+// [3] count, [4] pixels where o is zero and g is not.  One double partial per
+// block and sum; the host adds the blocks.
+constexpr int kCmpBlock = 256;
+__global__ void __launch_bounds__(kCmpBlock)
+k_compare(const float2* __restrict__ g, const float2* __restrict__ o, size_t n,
+          double* __restrict__ part) {
+  double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    const float2 a = g[i], b = o[i];
+    const double ox = b.x, oy = b.y, gx = a.x, gy = a.y;
+    if (ox != 0.0 || oy != 0.0) {
+      const double w = ox * ox + oy * oy;
+      const double ph = atan2(gy * ox - gx * oy, gx * ox + gy * oy);
+      const double da = sqrt(gx * gx + gy * gy) - sqrt(w);
+      acc[0] += w * ph * ph;
+      acc[1] += w;
+      acc[2] += da * da;
+      acc[3] += 1.0;
+    } else if (gx != 0.0 || gy != 0.0) {
+      acc[4] += 1.0;
+    }
+  }
+  __shared__ double red[5][kCmpBlock];
+  for (int k = 0; k < 5; ++k) red[k][threadIdx.x] = acc[k];
+  __syncthreads();
+  for (int h = kCmpBlock / 2; h > 0; h >>= 1) {
+    if ((int)threadIdx.x < h)
+      for (int k = 0; k < 5; ++k) red[k][threadIdx.x] += red[k][threadIdx.x + h];
+    __syncthreads();
+  }
+  if (threadIdx.x < 5) part[(size_t)blockIdx.x * 5 + threadIdx.x] = red[threadIdx.x][0];
 }
 
 __global__ void k_copy16(uint4* __restrict__ d, const uint4* __restrict__ s, size_t n16,
@@ -981,6 +1028,26 @@ hipError_t launch_zero(void* p, size_t bytes, hipStream_t s) {
   g = g == 0 ? 1 : (g < 65535 ? g : 65535);
   hipLaunchKernelGGL(k_zero16, dim3((unsigned)g), dim3(kBlock), 0, s,
                      (uint4*)p, n16, (uint8_t*)p + n16 * 16, ntail);
+  return hipGetLastError();
+}
+
+hipError_t launch_merge(float2* grid, const float2* out, size_t n, size_t grid_off,
+                        hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  size_t g = blocks_for(n, kBlock);
+  g = g < 16384 ? g : 16384;
+  hipLaunchKernelGGL(k_merge, dim3((unsigned)g), dim3(kBlock), 0, s, grid, out, n, grid_off);
+  return hipGetLastError();
+}
+
+int compare_blocks(size_t n) {
+  const size_t g = blocks_for(n ? n : 1, kCmpBlock);
+  return (int)(g < 4096 ? g : 4096);
+}
+
+hipError_t launch_compare(const float2* g, const float2* o, size_t n, double* part,
+                          int blocks, hipStream_t s) {
+  hipLaunchKernelGGL(k_compare, dim3(blocks), dim3(kCmpBlock), 0, s, g, o, n, part);
   return hipGetLastError();
 }
 

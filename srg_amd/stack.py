@@ -1,0 +1,209 @@
+"""Acquisition stacks on one grid: admission into HBM-sized batches and the
+.geo merge on the device.
+
+The reference builds a time series one scene at a time: process_parallel.py:28-80
+splits the list of acquisitions over the GPUs and runs each scene through
+sentinel_scene_multigpu.py:47-55, which creates the scene's .geo grid on the
+common DEM (createslc) and backprojects it burst by burst
+(sentinel/backprojectgpusub.f90:140-248), every azimuth_compress_ call merging
+its rows into the .geo file where |Re| > 1e-18
+(sentinel/azimuth_compress.cu:527-531).
+
+Here the bursts of every acquisition are one list in processing order
+(acquisition, then burst).  `plan_batches` admits consecutive bursts into
+batches whose device footprint fits a byte budget; each batch is one persistent
+launch (`tdbp.Batch`) followed by the device merges of its bursts into their
+acquisition grids in list order, so overlapping rows end with the later burst,
+as the reference's sequential read-modify-write leaves them.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+from typing import Callable, List, Optional, Sequence
+
+import numpy as np
+
+from . import tdbp
+
+# Device memory kept free beside the admitted bursts (driver, graph, queues).
+RESERVE_BYTES = 2 << 30
+
+
+def burst_device_bytes(geom: tdbp.BurstGeometry, dtype: int = tdbp.CF16,
+                       own_dem: bool = False) -> int:
+    """Device bytes one burst holds while its batch is resident: samples in
+    the storage type (4 B per sample for cf16/bf16, 8 for cf32) and their
+    per-pulse scales, pulse positions, parabola coefficients, the output
+    window, its own DEM window when it does not share the stack's, and the
+    batch's per-burst tables (orbit, pair counts, queue items)."""
+    elem = 8 if dtype == tdbp.CF32 else 4
+    b = geom.L * geom.S * elem
+    if dtype != tdbp.CF32:
+        b += geom.L * 4
+    b += geom.L * 24                       # satloc
+    b += geom.nlines * 24                  # coef
+    b += geom.nlines * geom.demwidth * 8   # output window (cf32)
+    b += geom.nlines * 8 + 4096            # pair counts, BurstDev, orbit, items
+    if own_dem:
+        b += geom.nlines * geom.demwidth * 2
+    return int(b)
+
+
+def plan_batches(sizes: Sequence[int], budget: int) -> List[List[int]]:
+    """Split bursts (in processing order) into consecutive batches whose
+    summed sizes fit `budget` bytes.  A burst larger than the budget on its own
+    cannot be admitted: ValueError."""
+    if budget <= 0:
+        raise ValueError(f"admission budget {budget} B")
+    batches: List[List[int]] = []
+    cur: List[int] = []
+    used = 0
+    for i, sz in enumerate(sizes):
+        if sz > budget:
+            raise ValueError(f"burst {i} needs {sz} B, more than the {budget} B budget")
+        if cur and used + sz > budget:
+            batches.append(cur)
+            cur, used = [], 0
+        cur.append(i)
+        used += sz
+    if cur:
+        batches.append(cur)
+    return batches
+
+
+def default_budget(fixed_bytes: int = 0) -> int:
+    """Free device memory now, less what the caller still has to allocate
+    (`fixed_bytes`: grids, staging) and a reserve."""
+    free, _ = tdbp.mem_info()
+    return int(free) - int(fixed_bytes) - RESERVE_BYTES
+
+
+@dataclass
+class StackBurst:
+    """One burst of a stack: its acquisition, geometry, host pulse positions
+    and DEM window (a host int16 [nlines][W] array, uploaded while the burst is
+    resident, or a device view of the stack's grid DEM, shared); the loader
+    fills its device samples."""
+
+    acq: int
+    geom: tdbp.BurstGeometry
+    satloc: np.ndarray
+    dem: object
+    payload: object = None
+
+    @property
+    def own_dem(self) -> bool:
+        return not isinstance(self.dem, tdbp.DeviceArray)
+
+
+@dataclass
+class Resident:
+    """Device buffers of an admitted burst."""
+
+    samples: tdbp.Samples
+    buffers: list                  # freed with the burst (samples, scales, dem)
+    sat: tdbp.DeviceArray
+    dem: tdbp.DeviceArray
+    out: tdbp.DeviceArray
+
+
+# loader(index, burst, resident) fills resident.samples' buffers on the device.
+Loader = Callable[[int, StackBurst, "Resident"], None]
+
+
+@dataclass
+class StackRun:
+    """Result of `run_stack`: the batches, the device time of every run of
+    each batch (run_ms[b][r]) and each batch's pixel x pulse pairs."""
+
+    batches: List[List[int]]
+    run_ms: List[List[float]] = field(default_factory=list)
+    pairs: List[int] = field(default_factory=list)
+
+
+def allocate(burst: StackBurst, dtype: int) -> Resident:
+    g = burst.geom
+    if dtype == tdbp.CF32:
+        data = tdbp.DeviceArray((g.L, g.S), np.complex64)
+        smp = tdbp.samples(data)
+        bufs = [data]
+    else:
+        data = tdbp.DeviceArray((g.L, g.S), np.uint32)
+        scale = tdbp.DeviceArray((g.L,), np.float32)
+        smp = tdbp.samples(data, dtype, scale)
+        bufs = [data, scale]
+    if burst.own_dem:
+        dem = tdbp.DeviceArray.from_host(np.ascontiguousarray(burst.dem, np.int16))
+        bufs.append(dem)
+    else:
+        dem = burst.dem
+        if dem.shape != (g.nlines, g.demwidth) or dem.dtype != np.int16:
+            raise ValueError(f"DEM view {dem.shape} {dem.dtype} for a "
+                             f"{g.nlines} x {g.demwidth} window")
+    return Resident(samples=smp, buffers=bufs, sat=tdbp.DeviceArray.from_host(burst.satloc),
+                    dem=dem, out=tdbp.DeviceArray((g.nlines, g.demwidth), np.complex64))
+
+
+def release(res: Resident):
+    for b in res.buffers + [res.sat, res.out]:
+        b.free()
+
+
+def run_stack(bursts: Sequence[StackBurst], grids: Sequence[tdbp.DeviceArray],
+              loader: Loader, *, precision: int = tdbp.MIXED, dtype: int = tdbp.CF16,
+              variant: int = -1, budget: Optional[int] = None, repeats: int = 1,
+              on_resident: Optional[Callable[[List[int], List[Resident]], None]] = None,
+              ) -> StackRun:
+    """Backproject every burst into its acquisition grid.
+
+    Bursts are admitted in order into batches that fit `budget` device bytes
+    (default: the free memory now, less a reserve).  Per batch: allocate the
+    bursts' buffers, call `loader` for each, build one persistent launch with a
+    merge of each burst into grids[burst.acq] at its first DEM line, run it
+    `repeats` times (device time of each run recorded), free the batch.
+    `on_resident(indices, residents)` is called after the loads, before the
+    batch runs.  The grids must be zeroed by the caller once before the stack.
+    """
+    if precision == tdbp.PLAIN_FP64 and dtype != tdbp.CF32:
+        raise ValueError("the FP64 kernel reads cf32 samples")
+    for b in bursts:
+        gr = grids[b.acq]
+        if gr.shape[1] != b.geom.demwidth or b.geom.firstline + b.geom.nlines > gr.shape[0]:
+            raise ValueError(f"burst window rows [{b.geom.firstline}, "
+                             f"{b.geom.firstline + b.geom.nlines}) x {b.geom.demwidth} "
+                             f"outside grid {gr.shape}")
+    sizes = [burst_device_bytes(b.geom, dtype, b.own_dem) for b in bursts]
+    if budget is None:
+        budget = default_budget()
+    run = StackRun(batches=plan_batches(sizes, budget))
+    stream = tdbp.Stream()
+    ev0, ev1 = tdbp.Event(), tdbp.Event()
+    for idx in run.batches:
+        res = [allocate(bursts[i], dtype) for i in idx]
+        try:
+            for i, r in zip(idx, res):
+                loader(i, bursts[i], r)
+            tdbp.sync()
+            if on_resident is not None:
+                on_resident(idx, res)
+            batch = tdbp.Batch(precision, variant)
+            try:
+                for k, (i, r) in enumerate(zip(idx, res)):
+                    g = bursts[i].geom
+                    batch.add(g, r.samples, r.sat, r.dem, r.out)
+                    batch.set_merge(k, grids[bursts[i].acq], g.firstline)
+                times = []
+                for _ in range(repeats):
+                    ev0.record(stream)
+                    batch.run(stream)
+                    ev1.record(stream)
+                    stream.sync()
+                    times.append(ev0.elapsed_ms(ev1))
+                run.run_ms.append(times)
+                run.pairs.append(batch.pairs)
+            finally:
+                batch.close()
+        finally:
+            for r in res:
+                release(r)
+    return run

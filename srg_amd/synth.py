@@ -276,28 +276,53 @@ def make_burst(L: int, S: int, W: int, nlines: int, *, centre_range: float = 850
     firstlat = lat_c - 0.5 * nlines * dlat
     firstlon = lon_c - 0.5 * W * dlon
     demlength = nlines
-    # latlons that reproduce exactly this window (rows 0..nlines-1, cols 0..W-1)
-    latlons = np.array([firstlat + (nlines - 0.5) * dlat, firstlat + 0.5 * dlat,
-                        firstlon + 0.5 * dlon, firstlon + (W - 0.5) * dlon])
 
     if dem == "flat":
         hgt = np.zeros((nlines, W), np.int16)
     else:
         src = dem_source if dem_source is not None else fbm_surface((1024, 1024), seed=seed)
-        r0, c0 = dem_offset
-        rr = (np.arange(nlines) + r0) % src.shape[0]
-        cc = (np.arange(W) + c0) % src.shape[1]
-        hgt = src[np.ix_(rr, cc)]
+        hgt = dem_window(src, dem_offset, nlines, W)
 
+    return _burst_on_window(
+        L=L, S=S, aperture=int(aperture), rngstart=rngstart, rngend=rngend, dmrg=dmrg,
+        wvl=wvl, prf=prf, angc0=angc0, angc1=angc1, tstart=tstart, tmid=tmid,
+        timeorbit=timeorbit, xx=xx, vv=vv, satloc=satloc, times=times, xyz_mid=xyz_mid,
+        vel_mid=vel_mid, orbit=orbit, firstlat=firstlat, firstlon=firstlon, dlat=dlat,
+        dlon=dlon, firstline=0, hgt=hgt, demlength=demlength, ntargets=ntargets,
+        clutter_sigma=clutter_sigma, seed=seed, target_rng=rng, fs=fs,
+        fit_last_line=fit_last_line)
+
+
+def dem_window(src: np.ndarray, offset, nlines: int, W: int) -> np.ndarray:
+    """nlines x W heights cut from a periodic source at (row, col) offset."""
+    r0, c0 = offset
+    rr = (np.arange(nlines) + r0) % src.shape[0]
+    cc = (np.arange(W) + c0) % src.shape[1]
+    return src[np.ix_(rr, cc)]
+
+
+def _burst_on_window(*, L, S, aperture, rngstart, rngend, dmrg, wvl, prf, angc0, angc1,
+                     tstart, tmid, timeorbit, xx, vv, satloc, times, xyz_mid, vel_mid,
+                     orbit, firstlat, firstlon, dlat, dlon, firstline, hgt, demlength,
+                     ntargets, clutter_sigma, seed, target_rng, fs,
+                     fit_last_line) -> BurstScene:
+    """Geometry, latlons and point targets of a burst whose DEM window is rows
+    [firstline, firstline + nlines) of a grid starting at (firstlat, firstlon)."""
+    nlines, W = hgt.shape
+    lat0 = firstlat + firstline * dlat
+    # latlons that reproduce exactly this window (rows firstline.., cols 0..W-1)
+    latlons = np.array([lat0 + (nlines - 0.5) * dlat, lat0 + 0.5 * dlat,
+                        firstlon + 0.5 * dlon, firstlon + (W - 0.5) * dlon])
     geom = tdbp.BurstGeometry(
         L=L, S=S, A=int(aperture), rngstart=rngstart, rngend=rngend, dmrg=dmrg, wvl=wvl,
         prf=prf, angc0=angc0, angc1=angc1, tmid=tmid, xyz_mid=xyz_mid, vel_mid=vel_mid,
-        timeorbit=timeorbit, xx=xx, vv=vv, demwidth=W, firstline=0, nlines=nlines,
+        timeorbit=timeorbit, xx=xx, vv=vv, demwidth=W, firstline=firstline, nlines=nlines,
         firstpix=0, lastpix=W - 1, firstlat=firstlat, firstlon=firstlon, deltalat=dlat,
         deltalon=dlon, fit_last_line=fit_last_line)
 
     # Point targets on DEM pixels whose zero-Doppler pulse lies well inside
     # the burst (the aperture centre is then valid), drawn near the centre.
+    rng = target_rng
     rows, cols = [], []
     margin = aperture // 2 + 8
     for _ in range(2000):
@@ -305,7 +330,7 @@ def make_burst(L: int, S: int, W: int, nlines: int, *, centre_range: float = 850
             break
         r = int(rng.integers(1, max(2, geom.nfit - 1)))
         c = int(rng.integers(W // 8, W - W // 8))
-        x = llh_to_ecef(firstlat + r * dlat, firstlon + c * dlon, float(hgt[r, c]))
+        x = llh_to_ecef(lat0 + r * dlat, firstlon + c * dlon, float(hgt[r, c]))
         p = (zero_doppler_time(orbit, x, tmid) - tstart) * prf
         if margin <= p <= L - 1 - margin:
             rows.append(r)
@@ -313,7 +338,7 @@ def make_burst(L: int, S: int, W: int, nlines: int, *, centre_range: float = 850
     rows = np.array(rows, np.int64)
     cols = np.array(cols, np.int64)
     ntargets = len(rows)
-    lat_t = firstlat + rows * dlat
+    lat_t = lat0 + rows * dlat
     lon_t = firstlon + cols * dlon
     txyz = llh_to_ecef(lat_t, lon_t, hgt[rows, cols].astype(np.float64)).reshape(-1, 3)
     tamp = np.exp(1j * rng.uniform(-np.pi, np.pi, ntargets))
@@ -375,7 +400,7 @@ def device_samples(sc: BurstScene, d_out: "tdbp.DeviceArray", d_satloc, ntaps: i
 # ---------------------------------------------------------- configurations
 
 
-def config1(seed: int = 0) -> BurstScene:
+def small_burst(seed: int = 0) -> BurstScene:
     """One synthetic burst on the reference CPU path: 512 pulses x 4096
     samples, 256 x 256 grid at 0.0002 deg on a flat DEM, A = 256, five unit
     point targets plus clutter sigma 0.1."""
@@ -383,7 +408,7 @@ def config1(seed: int = 0) -> BurstScene:
                       dem="flat", ntargets=5, clutter_sigma=0.1, seed=seed)
 
 
-def config2(seed: int = 0, dem_source=None) -> BurstScene:
+def iw_burst(seed: int = 0, dem_source=None) -> BurstScene:
     """One full IW burst: 1500 pulses x 21000 samples, 4000 x 1500 grid on an
     fBm DEM (0-3000 m, Hurst 0.8), A from the caller's formula."""
     src = dem_source if dem_source is not None else fbm_surface((2048, 2048), seed=seed + 7)
@@ -391,7 +416,7 @@ def config2(seed: int = 0, dem_source=None) -> BurstScene:
                       clutter_sigma=0.1, seed=seed)
 
 
-def config4(seed: int = 1, nsub: int = 3, nburst: int = 9) -> List[BurstScene]:
+def iw_scene(seed: int = 1, nsub: int = 3, nburst: int = 9) -> List[BurstScene]:
     """IW scene: 3 subswaths x 9 bursts of uneven size.  Pulses U[1350,1550],
     samples U[18000,25000], output tiles U[1000,4500] x U[1200,1600] on the
     fBm DEM; subswath centre ranges 825/875/925 km."""
@@ -414,3 +439,138 @@ def config4(seed: int = 1, nsub: int = 3, nburst: int = 9) -> List[BurstScene]:
                             t0=t0, orbit=orbit)
             scenes.append(sc)
     return scenes
+
+
+@dataclass
+class StackGrid:
+    """A common lat/lon DEM grid that every burst of a stack maps onto."""
+
+    firstlat: float
+    firstlon: float
+    dlat: float
+    dlon: float
+    dem: np.ndarray                    # int16 [lines][W]
+
+    @property
+    def lines(self):
+        return self.dem.shape[0]
+
+    @property
+    def W(self):
+        return self.dem.shape[1]
+
+
+def stack_burst(grid: StackGrid, orbit: Orbit, firstline: int, nlines: int, L: int,
+                S: int, tcentre: float, centre_range: float, *, seed: int,
+                target_seed: int, ntargets: int = 3, clutter_sigma: float = 0.1,
+                wvl: float = WVL_S1, fs: float = FS_IW, prf: float = PRF_IW) -> BurstScene:
+    """A burst of L pulses centred at tcentre whose DEM window is grid rows
+    [firstline, firstline + nlines), all W columns.  Targets are drawn from
+    target_seed, so bursts of several acquisitions over one window share them."""
+    dmrg = C_LIGHT / 2.0 / fs
+    dtaz = 1.0 / prf
+    tstart = tcentre - 0.5 * (L - 1) * dtaz
+    tend = tstart + (L - 1) * dtaz
+    tmid = 0.5 * (tstart + tend)
+    timeorbit, xx, vv = orbit.state_vectors(tmid)
+    times = tstart + dtaz * np.arange(L, dtype=np.float64)
+    satloc, satvel = tdbp.orbit_interp(timeorbit, xx, vv, times)
+    xyz_mid, vel_mid = tdbp.orbit_interp(timeorbit, xx, vv, [tmid])
+    rngstart = centre_range - 0.5 * S * dmrg
+    rngend = rngstart + (S - 1) * dmrg
+    aperture = aperture_from_caller(rngstart, wvl, prf, satvel[L // 2])
+    return _burst_on_window(
+        L=L, S=S, aperture=aperture, rngstart=rngstart, rngend=rngend, dmrg=dmrg, wvl=wvl,
+        prf=prf, angc0=0.0, angc1=0.0, tstart=tstart, tmid=tmid, timeorbit=timeorbit,
+        xx=xx, vv=vv, satloc=satloc, times=times, xyz_mid=xyz_mid[0], vel_mid=vel_mid[0],
+        orbit=orbit, firstlat=grid.firstlat, firstlon=grid.firstlon, dlat=grid.dlat,
+        dlon=grid.dlon, firstline=firstline, hgt=grid.dem[firstline:firstline + nlines],
+        demlength=grid.lines, ntargets=ntargets, clutter_sigma=clutter_sigma, seed=seed,
+        target_rng=np.random.default_rng(target_seed), fs=fs, fit_last_line=0)
+
+
+@dataclass
+class Stack:
+    """Acquisitions x bursts over one grid; bursts[a][k] is burst k of
+    acquisition a, baselines[a] its perpendicular orbit offset (ECEF, m)."""
+
+    grid: StackGrid
+    bursts: List[List[BurstScene]]
+    baselines: np.ndarray
+
+    def items(self):
+        """(acquisition, burst index, scene) in processing order."""
+        for a, row in enumerate(self.bursts):
+            for k, sc in enumerate(row):
+                yield a, k, sc
+
+
+def ts_stack(seed: int = 1, nacq: int = 30, nburst: int = 9, lines: int = 13500,
+            W: int = 4000, max_baseline: float = 200.0) -> Stack:
+    """Time-series stack: nacq acquisitions x nburst bursts over one common
+    lines x W grid on an fBm DEM.
+
+    Bursts are sized as in the IW scene (pulses U[1350,1550], samples
+    U[18000,25000], window rows U[1200,1600]); burst k's window is centred on
+    grid row (k + 1/2) lines/nburst, so neighbouring windows overlap or leave
+    gaps and the .geo merge order decides the overlap.  The row posting makes
+    the band of valid aperture centres of a 1450-pulse burst span lines/nburst
+    rows.  Every acquisition flies the reference orbit shifted by a seeded
+    perpendicular baseline of magnitude U[0, max_baseline] m at a seeded angle
+    about the along-track axis; the pulses of burst k are centred on the
+    reference orbit's zero-Doppler time of the window centre.  This is a
+    stripmap-like synthetic stack (no TOPS steering): what it fixes is the
+    work per burst, the grid and the merge, not the IW timeline.
+    """
+    rng = np.random.default_rng(seed)
+    centre_range = 875e3
+    t0 = 43200.0
+    orbit0 = make_orbit(35.0, -118.0, t0, seed=seed)
+    g0 = broadside_ground_point(orbit0, t0, centre_range)
+    lat_c, lon_c, _ = ecef_to_llh(g0)
+    lat_c, lon_c = float(lat_c), float(lon_c)
+    s0, v0 = orbit0.pos(t0), orbit0.vel(t0)
+    ground_step = np.linalg.norm(v0) / PRF_IW * (np.linalg.norm(g0) / np.linalg.norm(s0))
+    dmrg = C_LIGHT / 2.0 / FS_IW
+    a_ref = aperture_from_caller(centre_range - 0.5 * 21500 * dmrg, WVL_S1, PRF_IW, v0)
+    spacing = lines / nburst
+    dlat = -((1450 - a_ref) * ground_step / spacing) / 111320.0
+    dlon = 0.0002
+    src = fbm_surface((2048, 2048), seed=seed + 7)
+    grid = StackGrid(firstlat=lat_c - 0.5 * lines * dlat, firstlon=lon_c - 0.5 * W * dlon,
+                     dlat=dlat, dlon=dlon,
+                     dem=dem_window(src, (int(rng.integers(0, 2048)),
+                                          int(rng.integers(0, 2048))), lines, W))
+
+    # Burst windows and pulse centres, common to every acquisition.
+    windows, tcs = [], []
+    for k in range(nburst):
+        nl = int(rng.integers(1200, 1601))
+        crow = int(round((k + 0.5) * spacing))
+        fl = min(max(crow - nl // 2, 0), lines - nl)
+        x = llh_to_ecef(grid.firstlat + crow * dlat, grid.firstlon + 0.5 * W * dlon,
+                        float(grid.dem[crow, W // 2]))
+        windows.append((fl, nl))
+        tcs.append(zero_doppler_time(orbit0, x, t0))
+
+    vh = v0 / np.linalg.norm(v0)
+    up = s0 / np.linalg.norm(s0)
+    up = up - vh * np.dot(up, vh)
+    up /= np.linalg.norm(up)
+    side = np.cross(vh, up)
+    bursts, baselines = [], []
+    for a in range(nacq):
+        bmag = rng.uniform(0.0, max_baseline)
+        ang = rng.uniform(-math.pi, math.pi)
+        off = bmag * (math.cos(ang) * up + math.sin(ang) * side)
+        baselines.append(off)
+        orbit = orbit0.shifted(off)
+        row = []
+        for k, ((fl, nl), tc) in enumerate(zip(windows, tcs)):
+            L = int(rng.integers(1350, 1551))
+            S = int(rng.integers(18000, 25001))
+            row.append(stack_burst(grid, orbit, fl, nl, L, S, tc, centre_range,
+                                   seed=seed * 100000 + 100 * a + k,
+                                   target_seed=seed * 1000 + k))
+        bursts.append(row)
+    return Stack(grid=grid, bursts=bursts, baselines=np.array(baselines))

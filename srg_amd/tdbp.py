@@ -7,6 +7,7 @@ device is visible, the calls raise.
 from __future__ import annotations
 
 import ctypes as C
+import math
 import os
 from dataclasses import dataclass, field
 from typing import Optional
@@ -122,6 +123,9 @@ def _sig(lib):
         "tdbp_batch_create": (_VP, [C.c_int, C.c_int]),
         "tdbp_batch_add": (C.c_int, [_VP, _GP, _SP, _VP, _VP, _VP]),
         "tdbp_batch_run": (C.c_int, [_VP, _VP]),
+        "tdbp_batch_set_merge": (C.c_int, [_VP, C.c_int, _VP, C.c_int, C.c_int, C.c_int]),
+        "tdbp_compare": (C.c_int, [_VP, _VP, C.c_size_t, C.POINTER(C.c_double), _VP]),
+        "tdbp_merge_geo": (C.c_int, [_VP, C.c_int, C.c_int, C.c_int, C.c_int, _VP, _VP]),
         "tdbp_batch_pairs": (C.c_int64, [_VP]),
         "tdbp_batch_variant": (C.c_int, [_VP]),
         "tdbp_batch_device_bytes": (C.c_size_t, [_VP]),
@@ -154,7 +158,8 @@ def exported_symbols():
         "tdbp_pair_count", "tdbp_pack_samples", "tdbp_pixelint", "tdbp_backproject",
         "tdbp_backproject_variant", "tdbp_autotune", "tdbp_set_variant",
         "tdbp_variant_count", "tdbp_variant_info", "tdbp_batch_create", "tdbp_batch_add",
-        "tdbp_batch_run", "tdbp_batch_pairs", "tdbp_batch_variant",
+        "tdbp_batch_run", "tdbp_batch_set_merge", "tdbp_merge_geo", "tdbp_compare", "tdbp_batch_pairs",
+        "tdbp_batch_variant",
         "tdbp_batch_device_bytes", "tdbp_batch_destroy", "tdbp_synth_clutter",
         "tdbp_synth_targets", "azimuth_compress_",
     ]
@@ -197,6 +202,21 @@ class DeviceArray:
         if not p:
             check(ERR_NOMEM)
         self.ptr = p
+        self._base = None
+
+    def rows(self, first: int, count: int) -> "DeviceArray":
+        """View of rows [first, first + count) along the first axis; it shares
+        this allocation and frees nothing."""
+        if first < 0 or count < 1 or first + count > self.shape[0]:
+            raise ValueError(f"rows [{first}, {first + count}) of {self.shape}")
+        v = object.__new__(DeviceArray)
+        v.shape = (count,) + self.shape[1:]
+        v.dtype = self.dtype
+        row = self.nbytes // self.shape[0]
+        v.nbytes = row * count
+        v.ptr = self.ptr + row * first
+        v._base = self
+        return v
 
     @classmethod
     def from_host(cls, a: np.ndarray, stream=None):
@@ -219,9 +239,9 @@ class DeviceArray:
         check(lib().tdbp_dev_zero(self.ptr, self.nbytes, stream))
 
     def free(self):
-        if self.ptr:
+        if self.ptr and self._base is None:
             lib().tdbp_dev_free(self.ptr)
-            self.ptr = None
+        self.ptr = None
 
     def __del__(self):
         try:
@@ -411,6 +431,15 @@ def pack(d_cf32: DeviceArray, L: int, S: int, dtype: int, stream=None):
     return d_packed, d_scale
 
 
+def pack_into(d_cf32: DeviceArray, L: int, S: int, dtype: int, d_packed: DeviceArray,
+              d_scale: DeviceArray, stream=None):
+    """pack() into existing buffers (d_packed >= L*S words, d_scale >= L)."""
+    if d_packed.nbytes < L * S * 4 or d_scale.nbytes < L * 4 or d_cf32.nbytes < L * S * 8:
+        raise ValueError("pack_into: buffers smaller than L x S")
+    check(lib().tdbp_pack_samples(d_cf32.ptr, L, S, dtype, d_packed.ptr, d_scale.ptr,
+                                  _s(stream)))
+
+
 def backproject(g: BurstGeometry, smp: Samples, d_satloc, d_dem, d_out,
                 precision=PLAIN_FP64, d_coef=None, variant=None, stream=None):
     gc = g.to_c()
@@ -460,6 +489,10 @@ class Batch:
             check(-r)
         return r
 
+    def set_merge(self, burst: int, d_grid: DeviceArray, firstline: int):
+        check(lib().tdbp_batch_set_merge(self.ptr, burst, d_grid.ptr, d_grid.shape[0],
+                                         d_grid.shape[1], firstline))
+
     def run(self, stream=None):
         check(lib().tdbp_batch_run(self.ptr, _s(stream)))
 
@@ -485,6 +518,27 @@ class Batch:
             self.close()
         except Exception:
             pass
+
+
+def merge_geo(d_grid: DeviceArray, firstline: int, d_out: DeviceArray, stream=None):
+    """Merge a burst output [nlines][W] into grid rows [firstline, ...)."""
+    nl, W = d_out.shape
+    check(lib().tdbp_merge_geo(d_grid.ptr, W, d_grid.shape[0], firstline, nl, d_out.ptr,
+                               _s(stream)))
+
+
+def compare(d_img: DeviceArray, d_ref: DeviceArray, stream=None) -> dict:
+    """RMS phase (rad) and relative amplitude error of a cf32 image against a
+    cf32 reference over the reference's nonzero pixels, summed on the device."""
+    if d_img.nbytes != d_ref.nbytes or d_img.dtype != np.complex64:
+        raise ValueError("compare needs two cf32 arrays of one size")
+    n = d_img.nbytes // 8
+    sums = (C.c_double * 5)()
+    check(lib().tdbp_compare(d_img.ptr, d_ref.ptr, n, sums, _s(stream)))
+    w = sums[1]
+    return {"phase_rms": math.sqrt(sums[0] / w) if w > 0 else 0.0,
+            "amp_rms": math.sqrt(sums[2] / w) if w > 0 else 0.0,
+            "pixels": int(sums[3]), "extra_nonzero": int(sums[4])}
 
 
 def synth_clutter(d: DeviceArray, L: int, S: int, seed: int, sigma: float,

@@ -13,7 +13,7 @@ reference outputs:
   tiny.npz      setxyz / setazoff / pixelint and the full entry
                 azimuth_compress_cpu_ (DEM and .geo through file descriptors)
                 on a 64-pulse x 256-sample burst, 16 x 24 grid
-  config1.npz   pixelint on config 1 (512 x 4096, 256 x 256, A = 256): the
+  small_burst.npz   pixelint on the small burst (512 x 4096, 256 x 256, A = 256): the
                 computed rows only, with the input checksum
 
 Usage: python tests/golden/make_golden.py
@@ -126,8 +126,8 @@ def main():
                         azoff=azoff, out=out, geo=geo, latlons=latlons, firstlat_full=fl,
                         **geom_dict(g))
 
-    # 4. config 1 (inputs regenerated from the seed; checksum stored).
-    sc1 = synth.config1()
+    # 4. the small burst (inputs regenerated from the seed; checksum stored).
+    sc1 = synth.small_burst()
     b1 = synth.host_samples(sc1)
     g1 = sc1.geom
     xyz1, xyzfit1 = ref.setxyz(g1, sc1.dem)
@@ -135,7 +135,7 @@ def main():
     az1 = ref.setazoff(g1, coef1)
     out1 = ref.pixelint(g1, b1, sc1.satloc, xyz1, az1)
     rows = np.nonzero((az1 > 0).any(1))[0]
-    np.savez_compressed(os.path.join(OUT, "config1.npz"), rows=rows, out=out1[rows],
+    np.savez_compressed(os.path.join(OUT, "small_burst.npz"), rows=rows, out=out1[rows],
                         azoff=az1[rows], coef=coef1,
                         digest=np.array(input_digest(b1, sc1.satloc, sc1.dem)))
     print("golden vectors written to", OUT)

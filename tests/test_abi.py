@@ -60,7 +60,7 @@ def test_window_matches_oracle(oracle):
 
 
 def test_orbit_interp_bit_exact_with_oracle(oracle):
-    sc = synth.config1()
+    sc = synth.small_burst()
     g = sc.geom
     times = np.linspace(g.timeorbit[0] - 2, g.timeorbit[-1] + 2, 97)
     pos, vel = tdbp.orbit_interp(g.timeorbit, g.xx, g.vv, times)

@@ -1,15 +1,16 @@
-"""GPU tests at the full sizes of the benchmark configurations.
+"""GPU tests at the full sizes of the benchmark workloads.
 
-The oracle cannot sum a whole config-2 burst in seconds, so at these sizes:
+The oracle cannot sum a whole IW burst (1500 x 21000 -> 4000 x 1500 px) in
+seconds, so at these sizes:
   * the FP64 kernel is checked against the oracle on a sample of window rows
     (all columns), to the 1e-6 bounds of the small-size parity tests;
   * the mixed kernel (every storage type) is checked against the FP64 kernel
     on the whole output: phase <= 1 mrad and amplitude <= 2e-4 for cf32 and
-    cf16 (cf16 on white clutter sits at ~1.3e-4, see the design notes), bf16
+    cf16 (the cf16 rounding of white clutter alone gives ~1.3e-4), bf16
     reported with 5 mrad / 5e-3;
   * size-independent properties: linearity in the data, zero in -> zero out,
     bitwise determinism, point targets focus with their phase;
-  * the batch over the 27 uneven config-4 bursts equals one launch per burst
+  * the batch over the 27 uneven bursts of the IW scene equals one launch per burst
     bit for bit.
 """
 import numpy as np
@@ -34,8 +35,8 @@ def errors(g, o, mask=None):
 
 
 @pytest.fixture(scope="module")
-def c2(gpu):
-    sc = synth.config2()
+def iwb(gpu):
+    sc = synth.iw_burst()
     g = sc.geom
     d = {"sc": sc, "data": T.DeviceArray((g.L, g.S), np.complex64),
          "sat": T.DeviceArray.from_host(sc.satloc), "dem": T.DeviceArray.from_host(sc.dem),
@@ -56,18 +57,18 @@ def run(d, precision, smp=None, out=None):
     return out.download()
 
 
-def test_c2_plain_vs_oracle_rows(c2, oracle):
-    g = c2["sc"].geom
-    plain = run(c2, T.PLAIN_FP64)
-    data = c2["data"].download()
-    xyz, xyzfit = oracle.setxyz(g, c2["sc"].dem)
-    coef = c2["coef"].download()
+def test_iw_burst_plain_vs_oracle_rows(iwb, oracle):
+    g = iwb["sc"].geom
+    plain = run(iwb, T.PLAIN_FP64)
+    data = iwb["data"].download()
+    xyz, xyzfit = oracle.setxyz(g, iwb["sc"].dem)
+    coef = iwb["coef"].download()
     np.testing.assert_array_equal(coef, oracle.fitcoef(g, xyzfit))
     az = oracle.setazoff(g, coef)
     rows = np.nonzero((az > 0).any(1))[0]
     pick = rows[np.linspace(0, len(rows) - 1, 6).astype(int)]
     for r in pick:
-        out, _, _ = oracle.pixelint(g, data, c2["sc"].satloc, xyz, az,
+        out, _, _ = oracle.pixelint(g, data, iwb["sc"].satloc, xyz, az,
                                     semantics=pyoracle.GPU_SEMANTICS, rows=slice(r, r + 1))
         m = az[r] > 0
         ph, amp = errors(plain[r][m], out[0][m])
@@ -77,48 +78,48 @@ def test_c2_plain_vs_oracle_rows(c2, oracle):
 
 @pytest.mark.parametrize("dtype,pb,ab", [(T.CF32, 1e-3, 2e-4), (T.CF16, 1e-3, 2e-4),
                                          (T.CBF16, 5e-3, 5e-3)])
-def test_c2_mixed_vs_plain(c2, dtype, pb, ab):
-    g = c2["sc"].geom
-    plain = run(c2, T.PLAIN_FP64)
+def test_iw_burst_mixed_vs_plain(iwb, dtype, pb, ab):
+    g = iwb["sc"].geom
+    plain = run(iwb, T.PLAIN_FP64)
     if dtype == T.CF32:
-        smp = T.samples(c2["data"])
+        smp = T.samples(iwb["data"])
     else:
-        pk, scl = T.pack(c2["data"], g.L, g.S, dtype)
+        pk, scl = T.pack(iwb["data"], g.L, g.S, dtype)
         smp = T.samples(pk, dtype, scl)
-    mixed = run(c2, T.MIXED, smp)
+    mixed = run(iwb, T.MIXED, smp)
     m = plain != 0
     ph, amp = errors(mixed, plain, m)
-    print(f"config 2 mixed {T.DTYPE_NAMES[dtype]} vs FP64: {ph*1e3:.4f} mrad, amp {amp:.2e}")
+    print(f"IW burst mixed {T.DTYPE_NAMES[dtype]} vs FP64: {ph*1e3:.4f} mrad, amp {amp:.2e}")
     assert ph <= pb and amp <= ab
     assert np.all(mixed[~m] == 0)
 
 
-def test_c2_linearity_zero_determinism(c2):
-    g = c2["sc"].geom
-    d1 = c2["data"].download()
+def test_iw_burst_linearity_zero_determinism(iwb):
+    g = iwb["sc"].geom
+    d1 = iwb["data"].download()
     rng = np.random.default_rng(5)
     d2 = (rng.standard_normal(d1.shape, np.float32) +
           1j * rng.standard_normal(d1.shape, np.float32)).astype(np.complex64)
     dev2 = T.DeviceArray.from_host(d2)
-    o1 = run(c2, T.PLAIN_FP64)
-    o1b = run(c2, T.PLAIN_FP64)
+    o1 = run(iwb, T.PLAIN_FP64)
+    o1b = run(iwb, T.PLAIN_FP64)
     assert np.array_equal(o1.view(np.uint32), o1b.view(np.uint32))
-    o2 = run(c2, T.PLAIN_FP64, T.samples(dev2))
+    o2 = run(iwb, T.PLAIN_FP64, T.samples(dev2))
     comb = T.DeviceArray.from_host((0.5 * d1 - 2j * d2).astype(np.complex64))
-    o3 = run(c2, T.PLAIN_FP64, T.samples(comb))
+    o3 = run(iwb, T.PLAIN_FP64, T.samples(comb))
     want = 0.5 * o1.astype(np.complex128) - 2j * o2.astype(np.complex128)
     scale = np.abs(want).max()
     assert np.max(np.abs(o3 - want)) <= 1e-5 * scale
     zero = T.DeviceArray((g.L, g.S), np.complex64)
     zero.zero()
     for prec in (T.PLAIN_FP64, T.MIXED):
-        oz = run(c2, prec, T.samples(zero))
+        oz = run(iwb, prec, T.samples(zero))
         assert np.all(oz == 0)
 
 
-def test_c2_targets_focus_with_their_phase(c2):
-    sc = c2["sc"]
-    plain = run(c2, T.PLAIN_FP64)
+def test_iw_burst_targets_focus_with_their_phase(iwb):
+    sc = iwb["sc"]
+    plain = run(iwb, T.PLAIN_FP64)
     for (r, c), a in zip(sc.targets_px, sc.targets_amp):
         v = plain[r, c]
         assert v != 0
@@ -127,8 +128,8 @@ def test_c2_targets_focus_with_their_phase(c2):
         assert abs(v) > 5 * np.median(np.abs(plain[r, max(c - 40, 0):c + 40]))
 
 
-def test_c4_batch_equals_single_bursts(gpu):
-    scenes = synth.config4()
+def test_iw_scene_batch_equals_single_bursts(gpu):
+    scenes = synth.iw_scene()
     variant = 5
     devs, outs, singles = [], [], []
     for sc in scenes:

@@ -69,7 +69,7 @@ class Dev:
 
 @pytest.fixture(scope="module")
 def c1(gpu, oracle):
-    sc = synth.config1()
+    sc = synth.small_burst()
     d = Dev(sc)
     out, acc, pairs, coef, azoff = oracle.backproject(sc.geom, d.data_h, sc.satloc, sc.dem)
     return d, dict(out=out, acc=acc, pairs=pairs, coef=coef, azoff=azoff)

@@ -1,0 +1,148 @@
+"""GPU tests of the stack path: the device .geo merge against the oracle's
+merge bit for bit, the device error sums against numpy, and a reduced
+time-series stack (2 acquisitions x 3 bursts on a 4500 x 1500 grid) through
+admission: several batches == one batch == one launch per burst merged in
+order, bit for bit, and the mixed cf16 grids within 1 mrad of the FP64 grids
+per acquisition."""
+import math
+
+import numpy as np
+import pytest
+
+from oracle import pyoracle
+from srg_amd import stack as ST
+from srg_amd import synth
+from srg_amd import tdbp as T
+
+pytestmark = pytest.mark.gpu
+
+VARIANT = 5
+
+
+def test_merge_geo_matches_oracle(gpu, oracle):
+    rng = np.random.default_rng(3)
+    grid = (rng.standard_normal((300, 257)) + 1j * rng.standard_normal((300, 257))).astype(
+        np.complex64)
+    d_grid = T.DeviceArray.from_host(grid)
+    host = grid.copy()
+    for fl, nl in ((0, 120), (100, 90), (170, 130)):
+        out = (rng.standard_normal((nl, 257)) + 1j * rng.standard_normal((nl, 257))).astype(
+            np.complex64)
+        out[rng.random(out.shape) < 0.3] = 0
+        planted = np.array([0.5 + 1j, 1e-19 + 2j, 0 + 5j, complex(math.nan, 1), 2e-18,
+                            -1e-17 + 3j], np.complex64)
+        out[nl // 2, :len(planted)] = planted
+        T.merge_geo(d_grid, fl, T.DeviceArray.from_host(out))
+        oracle.merge_geo(host, out, fl, pyoracle.GPU_SEMANTICS)
+    got = d_grid.download()
+    assert np.array_equal(got.view(np.uint32), host.view(np.uint32))
+    with pytest.raises(T.TdbpError):
+        T.merge_geo(d_grid, 250, T.DeviceArray((60, 257), np.complex64))
+
+
+def test_compare_matches_numpy(gpu):
+    rng = np.random.default_rng(4)
+    o = (rng.standard_normal(100003) + 1j * rng.standard_normal(100003)).astype(np.complex64)
+    o[rng.random(o.shape) < 0.2] = 0
+    g = (o * np.exp(1j * 1e-3 * rng.standard_normal(o.shape)) *
+         (1 + 1e-4 * rng.standard_normal(o.shape))).astype(np.complex64)
+    g[o == 0] = 0
+    g[:7][o[:7] == 0] = 1 + 1j        # extra nonzero pixels where o is zero
+    e = T.compare(T.DeviceArray.from_host(g), T.DeviceArray.from_host(o))
+    m = o != 0
+    gg, oo = g[m].astype(np.complex128), o[m].astype(np.complex128)
+    w = np.abs(oo) ** 2
+    ph = math.sqrt(np.sum(w * np.angle(gg * np.conj(oo)) ** 2) / np.sum(w))
+    amp = math.sqrt(np.sum((np.abs(gg) - np.abs(oo)) ** 2) / np.sum(w))
+    assert e["pixels"] == int(m.sum())
+    assert e["extra_nonzero"] == int(np.sum((o == 0) & (g != 0)))
+    assert abs(e["phase_rms"] - ph) <= 1e-9 * ph
+    assert abs(e["amp_rms"] - amp) <= 1e-9 * amp
+
+
+@pytest.fixture(scope="module")
+def small_stack(gpu):
+    stk = synth.ts_stack(nacq=2, nburst=3, lines=4500, W=1500)
+    d_dem = T.DeviceArray.from_host(stk.grid.dem)
+    bursts = [ST.StackBurst(a, sc.geom, sc.satloc,
+                            d_dem.rows(sc.geom.firstline, sc.geom.nlines), payload=sc)
+              for a, k, sc in stk.items()]
+    return {"stk": stk, "dem": d_dem, "bursts": bursts}
+
+
+def loader_for(dtype):
+    def load(i, b, r):
+        g = b.geom
+        if dtype == T.CF32:
+            synth.device_samples(b.payload, r.buffers[0], r.sat)
+        else:
+            tmp = T.DeviceArray((g.L, g.S), np.complex64)
+            synth.device_samples(b.payload, tmp, r.sat)
+            T.pack_into(tmp, g.L, g.S, dtype, r.buffers[0], r.buffers[1])
+            T.sync()
+            tmp.free()
+    return load
+
+
+def grids(stk):
+    gs = [T.DeviceArray((stk.grid.lines, stk.grid.W), np.complex64)
+          for _ in range(len(stk.bursts))]
+    for x in gs:
+        x.zero()
+    return gs
+
+
+def run(ss, precision, dtype, budget=None):
+    gs = grids(ss["stk"])
+    r = ST.run_stack(ss["bursts"], gs, loader_for(dtype), precision=precision, dtype=dtype,
+                     variant=VARIANT, budget=budget)
+    T.sync()
+    return [x.download() for x in gs], r
+
+
+def one_launch_per_burst(ss, precision, dtype, oracle):
+    """Reference ordering: each burst alone (tdbp_backproject), merged into a
+    host grid by the oracle's GPU-entry rule, in stack order."""
+    stk = ss["stk"]
+    host = [np.zeros((stk.grid.lines, stk.grid.W), np.complex64) for _ in stk.bursts]
+    for b in ss["bursts"]:
+        res = ST.allocate(b, dtype)
+        loader_for(dtype)(0, b, res)
+        T.sync()
+        T.backproject(b.geom, res.samples, res.sat, res.dem, res.out, precision,
+                      variant=VARIANT)
+        T.sync()
+        oracle.merge_geo(host[b.acq], res.out.download(), b.geom.firstline,
+                         pyoracle.GPU_SEMANTICS)
+        ST.release(res)
+    return host
+
+
+def test_stack_admission_batches_equal(small_stack, oracle):
+    one, r1 = run(small_stack, T.MIXED, T.CF16)
+    assert len(r1.batches) == 1
+    sizes = [ST.burst_device_bytes(b.geom, T.CF16, b.own_dem) for b in small_stack["bursts"]]
+    budget = max(sizes[0] + sizes[1], max(sizes))
+    many, r2 = run(small_stack, T.MIXED, T.CF16, budget=budget)
+    assert len(r2.batches) >= 3
+    assert sum(r2.pairs) == sum(r1.pairs) > 0
+    for a, b in zip(one, many):
+        assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+    seq = one_launch_per_burst(small_stack, T.MIXED, T.CF16, oracle)
+    for a, b in zip(one, seq):
+        assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+    for g in one:
+        assert np.count_nonzero(g) > 0.5 * g.size
+
+
+def test_stack_fp64_and_mixed_error_per_acquisition(small_stack, oracle):
+    fp64, r = run(small_stack, T.PLAIN_FP64, T.CF32)
+    seq = one_launch_per_burst(small_stack, T.PLAIN_FP64, T.CF32, oracle)
+    for a, b in zip(fp64, seq):
+        assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+    mixed, _ = run(small_stack, T.MIXED, T.CF16)
+    for a, (gm, gr) in enumerate(zip(mixed, fp64)):
+        e = T.compare(T.DeviceArray.from_host(gm), T.DeviceArray.from_host(gr))
+        print(f"acquisition {a}: {e['phase_rms'] * 1e3:.4f} mrad, amp {e['amp_rms']:.2e}")
+        assert e["phase_rms"] <= 1e-3 and e["amp_rms"] <= 2e-4
+        assert e["extra_nonzero"] == 0

@@ -148,9 +148,9 @@ def test_pixelint_cpu_semantics_golden(oracle):
     assert pairs > 0
 
 
-def test_pixelint_config1_golden(oracle):
-    d = load("config1.npz")
-    sc = synth.config1()
+def test_pixelint_small_burst_golden(oracle):
+    d = load("small_burst.npz")
+    sc = synth.small_burst()
     b = synth.host_samples(sc)
     import hashlib
 

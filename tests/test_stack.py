@@ -1,0 +1,137 @@
+"""Host-side tests of the stack path: the .geo merge rule of the oracle on the
+values where the two reference rules differ, the admission planner, the
+per-burst device footprint and the time-series stack geometry."""
+import math
+
+import numpy as np
+import pytest
+from hypothesis import given, settings
+from hypothesis import strategies as st
+
+from oracle import pyoracle
+from srg_amd import stack as ST
+from srg_amd import synth
+from srg_amd import tdbp as T
+
+
+def merged(oracle, vals, semantics):
+    grid = np.full((1, len(vals)), 7 + 7j, np.complex64)
+    out = np.array([vals], np.complex64)
+    oracle.merge_geo(grid, out, 0, semantics)
+    return grid[0]
+
+
+def test_merge_rule_gpu_entry(oracle):
+    # fabs((double)re) > 1e-18 (sentinel/azimuth_compress.cu:527-531): any
+    # real part of magnitude above 1e-18 replaces the pixel; the imaginary part
+    # is not looked at; NaN compares false.
+    vals = [0.5 + 1j, 1e-19 + 2j, -3 + 4j, 0 + 5j, complex(math.nan, 1), -0.9 + 0j,
+            2e-18 + 0j, -1e-17 + 3j]
+    take = [True, False, True, False, False, True, True, True]
+    g = merged(oracle, vals, pyoracle.GPU_SEMANTICS)
+    want = np.where(take, np.array(vals, np.complex64), np.complex64(7 + 7j))
+    assert np.array_equal(g.view(np.uint32), want.view(np.uint32))
+
+
+def test_merge_rule_cpu_twin(oracle):
+    # abs((int)re) > 1e-18 (sentinel/azimuth_compress_cpu.c:387): C's integer
+    # abs of the truncated real part, so only |re| >= 1 replaces the pixel.
+    vals = [0.5 + 1j, 1e-19 + 2j, -3 + 4j, 0 + 5j, -0.9 + 0j, 2e-18 + 0j, 1.0 + 0j,
+            -1.5 - 1j]
+    take = [False, False, True, False, False, False, True, True]
+    g = merged(oracle, vals, pyoracle.CPU_SEMANTICS)
+    want = np.where(take, np.array(vals, np.complex64), np.complex64(7 + 7j))
+    assert np.array_equal(g.view(np.uint32), want.view(np.uint32))
+
+
+def test_merge_later_burst_wins_overlap(oracle):
+    grid = np.zeros((10, 4), np.complex64)
+    a = np.full((6, 4), 1 + 1j, np.complex64)
+    a[0] = 0                              # a row outside the valid band
+    b = np.full((5, 4), 2 + 2j, np.complex64)
+    b[-1] = 0
+    oracle.merge_geo(grid, a, 0)
+    oracle.merge_geo(grid, b, 4)
+    assert np.all(grid[0] == 0)
+    assert np.all(grid[1:4] == 1 + 1j)
+    assert np.all(grid[4:8] == 2 + 2j)    # overlap rows 4, 5: the later burst
+    assert np.all(grid[8:] == 0)          # b's zero row keeps the grid's zero
+
+
+def test_plan_batches_examples():
+    assert ST.plan_batches([5, 5, 5], 100) == [[0, 1, 2]]
+    assert ST.plan_batches([5, 5, 5], 10) == [[0, 1], [2]]
+    assert ST.plan_batches([5, 5, 5], 5) == [[0], [1], [2]]
+    assert ST.plan_batches([], 5) == []
+    with pytest.raises(ValueError):
+        ST.plan_batches([5, 11], 10)
+    with pytest.raises(ValueError):
+        ST.plan_batches([1], 0)
+
+
+@settings(max_examples=200, deadline=None)
+@given(st.lists(st.integers(1, 1000), max_size=60), st.integers(1000, 5000))
+def test_plan_batches_properties(sizes, budget):
+    plan = ST.plan_batches(sizes, budget)
+    flat = [i for b in plan for i in b]
+    assert flat == list(range(len(sizes)))              # every burst once, in order
+    assert all(sum(sizes[i] for i in b) <= budget for b in plan)
+    for b, nxt in zip(plan, plan[1:]):                  # greedy: the next burst did not fit
+        assert sum(sizes[i] for i in b) + sizes[nxt[0]] > budget
+
+
+def test_burst_device_bytes():
+    sc = synth.small_burst()
+    g = sc.geom
+    b32 = ST.burst_device_bytes(g, T.CF32)
+    b16 = ST.burst_device_bytes(g, T.CF16)
+    assert b32 - b16 == g.L * g.S * 4 - g.L * 4
+    assert b16 >= g.L * g.S * 4 + g.nlines * g.demwidth * 8
+    assert ST.burst_device_bytes(g, T.CF16, own_dem=True) - b16 == g.nlines * g.demwidth * 2
+
+
+@pytest.fixture(scope="module")
+def stack():
+    return synth.ts_stack()
+
+
+def test_stack_shape(stack):
+    g = stack.grid
+    assert (g.lines, g.W) == (13500, 4000)
+    assert len(stack.bursts) == 30 and all(len(r) == 9 for r in stack.bursts)
+    for a, k, sc in stack.items():
+        geo = sc.geom
+        assert 1350 <= geo.L <= 1550 and 18000 <= geo.S <= 25000
+        assert 1200 <= geo.nlines <= 1600 and geo.demwidth == 4000
+        assert 0 <= geo.firstline and geo.firstline + geo.nlines <= g.lines
+        assert geo.firstlat == g.firstlat and geo.deltalat == g.dlat
+        assert np.array_equal(sc.dem, g.dem[geo.firstline:geo.firstline + geo.nlines])
+        assert geo.A < geo.L
+
+
+def test_stack_windows_and_targets_common(stack):
+    ref = stack.bursts[0]
+    for row in stack.bursts[1:]:
+        for s0, s1 in zip(ref, row):
+            assert (s0.geom.firstline, s0.geom.nlines) == (s1.geom.firstline, s1.geom.nlines)
+            assert np.array_equal(s0.targets_xyz, s1.targets_xyz)
+    # windows walk down the grid; some neighbours overlap, so the merge order
+    # decides those rows
+    fl = [sc.geom.firstline for sc in ref]
+    end = [sc.geom.firstline + sc.geom.nlines for sc in ref]
+    assert fl == sorted(fl)
+    assert any(f1 < e0 for e0, f1 in zip(end, fl[1:]))
+
+
+def test_stack_baselines_perpendicular(stack):
+    b = np.linalg.norm(stack.baselines, axis=1)
+    assert np.all(b <= 200.0) and b.max() > 100.0 and len(set(np.round(b, 6))) == 30
+    orbit = synth.make_orbit(35.0, -118.0, 43200.0, seed=1)
+    v = orbit.vel(43200.0)
+    cosang = stack.baselines @ v / (b * np.linalg.norm(v))
+    assert np.max(np.abs(cosang)) < 1e-12
+    # each acquisition's pulses sit on its own shifted orbit
+    for a in (0, 7, 29):
+        sc = stack.bursts[a][4]
+        p = orbit.pos(sc.times) + stack.baselines[a]
+        assert np.max(np.abs(sc.satloc - p)) < 1e-3
