@@ -100,6 +100,14 @@ void azimuth_compress_(void *burstdata, double *satloc, int *rawdatalines,
  * entry returns nothing). */
 int tdbp_dropin_status(void);
 
+/* Phase times (ms) of the last azimuth_compress_ call: [0] total, [1] DEM
+ * read, [2] DEM + satloc + burst upload, [3] device compute (pack, fit,
+ * sum), [4] .geo read (overlaps [3]), [5] .geo upload + merge + download,
+ * [6] .geo write.  Fills min(n, 7) values, returns 7.  No reference
+ * counterpart (the reference prints its timers when enabled,
+ * sentinel/azimuth_compress.cu:492-505, commented out). */
+int tdbp_dropin_timing(double *ms, int n);
+
 /* ------------------------------------------------------------- runtime */
 
 const char *tdbp_last_error(void);

@@ -1,0 +1,101 @@
+"""The drop-in entry azimuth_compress_ called the way the reference's Fortran
+burst driver calls it (sentinel/backprojectgpusub.f90:244-247): 33 arguments,
+all by reference, fixed-size Fortran arrays for satloc (3 x 100000), the
+state vectors (1000 and 3 x 1000) and the pulse times (100000), the DEM and
+the .geo grid as open POSIX file descriptors.
+
+`BurstCall` holds those arrays for one burst; `call()` makes the call and
+raises if the entry recorded an error (the entry itself returns nothing, as
+the reference's does).
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import tdbp
+
+
+@dataclass
+class BurstCall:
+    """Arguments of one azimuth_compress_ call, in Fortran layout."""
+
+    burst: np.ndarray          # complex64 [L][S] (Fortran (S, L))
+    satloc: np.ndarray         # float64 [100000][3]
+    L: int
+    S: int
+    W: int
+    demlength: int
+    deltalat: float
+    deltalon: float
+    firstlat: float
+    firstlon: float
+    latlons: np.ndarray        # float64 [4]: min lat, max lat, min lon, max lon
+    timeorbit: np.ndarray      # float64 [1000]
+    xx: np.ndarray             # float64 [1000][3]
+    vv: np.ndarray             # float64 [1000][3]
+    numstatevec: int
+    rngstart: float
+    rngend: float
+    tmid: float
+    xyz_mid: np.ndarray
+    vel_mid: np.ndarray
+    dmrg: float
+    wvl: float
+    aperture: int
+    angc0: float
+    angc1: float
+    prf: float
+
+    @classmethod
+    def from_burst(cls, data: np.ndarray, g: tdbp.BurstGeometry, satloc: np.ndarray,
+                   demlength: int, firstlat: float, latlons) -> "BurstCall":
+        """From a burst's samples and geometry; firstlat/demlength/latlons
+        describe the full DEM file the window is cut from."""
+        sat = np.zeros((100000, 3))
+        sat[:g.L] = satloc
+        n = len(g.timeorbit)
+        tt = np.zeros(1000)
+        tt[:n] = g.timeorbit
+        xx = np.zeros((1000, 3))
+        xx[:n] = g.xx
+        vv = np.zeros((1000, 3))
+        vv[:n] = g.vv
+        return cls(burst=np.ascontiguousarray(data, np.complex64), satloc=sat, L=g.L, S=g.S,
+                   W=g.demwidth, demlength=demlength, deltalat=g.deltalat,
+                   deltalon=g.deltalon, firstlat=firstlat, firstlon=g.firstlon,
+                   latlons=np.ascontiguousarray(latlons, np.float64), timeorbit=tt, xx=xx,
+                   vv=vv, numstatevec=n, rngstart=g.rngstart, rngend=g.rngend, tmid=g.tmid,
+                   xyz_mid=np.ascontiguousarray(g.xyz_mid, np.float64),
+                   vel_mid=np.ascontiguousarray(g.vel_mid, np.float64), dmrg=g.dmrg,
+                   wvl=g.wvl, aperture=g.A, angc0=g.angc0, angc1=g.angc1, prf=g.prf)
+
+    def call(self, fdout: int, fddem: int):
+        """azimuth_compress_ on this burst; raises TdbpError on a recorded
+        failure (the .geo is then unchanged)."""
+        i = lambda v: C.byref(C.c_int(int(v)))  # noqa: E731
+        d = lambda v: C.byref(C.c_double(float(v)))  # noqa: E731
+        dp = lambda a: a.ctypes.data_as(C.POINTER(C.c_double))  # noqa: E731
+        tp = np.zeros(100000)
+        tdbp.lib().azimuth_compress_(
+            self.burst.ctypes.data, dp(self.satloc), i(self.L), i(self.S), i(self.W),
+            i(self.demlength), i(fdout), i(fddem), d(self.deltalat), d(self.deltalon),
+            d(self.firstlat), d(self.firstlon), dp(self.latlons), dp(self.timeorbit),
+            dp(self.xx), dp(self.vv), i(self.numstatevec), d(self.rngstart), d(self.rngend),
+            d(0), d(0), d(self.tmid), dp(self.xyz_mid), dp(self.vel_mid), dp(tp),
+            d(1 / self.prf), d(self.dmrg), d(self.wvl), i(self.aperture), i(0),
+            d(self.angc0), d(self.angc1), d(self.prf))
+        tdbp.check(tdbp.lib().tdbp_dropin_status())
+
+
+TIMING_NAMES = ("total", "dem_read", "upload", "compute", "geo_read", "geo_merge",
+                "geo_write")
+
+
+def last_timing() -> dict:
+    """Phase times (ms) of the last azimuth_compress_ call."""
+    ms = (C.c_double * 7)()
+    tdbp.lib().tdbp_dropin_timing(ms, 7)
+    return dict(zip(TIMING_NAMES, list(ms)))

@@ -463,7 +463,8 @@ class StackGrid:
 def stack_burst(grid: StackGrid, orbit: Orbit, firstline: int, nlines: int, L: int,
                 S: int, tcentre: float, centre_range: float, *, seed: int,
                 target_seed: int, ntargets: int = 3, clutter_sigma: float = 0.1,
-                wvl: float = WVL_S1, fs: float = FS_IW, prf: float = PRF_IW) -> BurstScene:
+                wvl: float = WVL_S1, fs: float = FS_IW, prf: float = PRF_IW,
+                aperture: Optional[int] = None) -> BurstScene:
     """A burst of L pulses centred at tcentre whose DEM window is grid rows
     [firstline, firstline + nlines), all W columns.  Targets are drawn from
     target_seed, so bursts of several acquisitions over one window share them."""
@@ -478,7 +479,8 @@ def stack_burst(grid: StackGrid, orbit: Orbit, firstline: int, nlines: int, L: i
     xyz_mid, vel_mid = tdbp.orbit_interp(timeorbit, xx, vv, [tmid])
     rngstart = centre_range - 0.5 * S * dmrg
     rngend = rngstart + (S - 1) * dmrg
-    aperture = aperture_from_caller(rngstart, wvl, prf, satvel[L // 2])
+    if aperture is None:
+        aperture = aperture_from_caller(rngstart, wvl, prf, satvel[L // 2])
     return _burst_on_window(
         L=L, S=S, aperture=aperture, rngstart=rngstart, rngend=rngend, dmrg=dmrg, wvl=wvl,
         prf=prf, angc0=0.0, angc1=0.0, tstart=tstart, tmid=tmid, timeorbit=timeorbit,
@@ -506,7 +508,9 @@ class Stack:
 
 
 def ts_stack(seed: int = 1, nacq: int = 30, nburst: int = 9, lines: int = 13500,
-            W: int = 4000, max_baseline: float = 200.0) -> Stack:
+            W: int = 4000, max_baseline: float = 200.0, pulses=(1350, 1550),
+            samples=(18000, 25000), rows=(1200, 1600), aperture: Optional[int] = None,
+            ntargets: int = 3) -> Stack:
     """Time-series stack: nacq acquisitions x nburst bursts over one common
     lines x W grid on an fBm DEM.
 
@@ -515,7 +519,9 @@ def ts_stack(seed: int = 1, nacq: int = 30, nburst: int = 9, lines: int = 13500,
     grid row (k + 1/2) lines/nburst, so neighbouring windows overlap or leave
     gaps and the .geo merge order decides the overlap.  The row posting makes
     the band of valid aperture centres of a 1450-pulse burst span lines/nburst
-    rows.  Every acquisition flies the reference orbit shifted by a seeded
+    rows (the ranges are parameters; `aperture` fixes A instead of the
+    caller's formula, for bursts shorter than that formula's A).  Every
+    acquisition flies the reference orbit shifted by a seeded
     perpendicular baseline of magnitude U[0, max_baseline] m at a seeded angle
     about the along-track axis; the pulses of burst k are centred on the
     reference orbit's zero-Doppler time of the window centre.  This is a
@@ -532,9 +538,11 @@ def ts_stack(seed: int = 1, nacq: int = 30, nburst: int = 9, lines: int = 13500,
     s0, v0 = orbit0.pos(t0), orbit0.vel(t0)
     ground_step = np.linalg.norm(v0) / PRF_IW * (np.linalg.norm(g0) / np.linalg.norm(s0))
     dmrg = C_LIGHT / 2.0 / FS_IW
-    a_ref = aperture_from_caller(centre_range - 0.5 * 21500 * dmrg, WVL_S1, PRF_IW, v0)
+    l_mid = 0.5 * (pulses[0] + pulses[1])
+    a_ref = aperture if aperture is not None else aperture_from_caller(
+        centre_range - 0.25 * (samples[0] + samples[1]) * dmrg, WVL_S1, PRF_IW, v0)
     spacing = lines / nburst
-    dlat = -((1450 - a_ref) * ground_step / spacing) / 111320.0
+    dlat = -((l_mid - a_ref) * ground_step / spacing) / 111320.0
     dlon = 0.0002
     src = fbm_surface((2048, 2048), seed=seed + 7)
     grid = StackGrid(firstlat=lat_c - 0.5 * lines * dlat, firstlon=lon_c - 0.5 * W * dlon,
@@ -545,7 +553,7 @@ def ts_stack(seed: int = 1, nacq: int = 30, nburst: int = 9, lines: int = 13500,
     # Burst windows and pulse centres, common to every acquisition.
     windows, tcs = [], []
     for k in range(nburst):
-        nl = int(rng.integers(1200, 1601))
+        nl = int(rng.integers(rows[0], rows[1] + 1))
         crow = int(round((k + 0.5) * spacing))
         fl = min(max(crow - nl // 2, 0), lines - nl)
         x = llh_to_ecef(grid.firstlat + crow * dlat, grid.firstlon + 0.5 * W * dlon,
@@ -567,10 +575,11 @@ def ts_stack(seed: int = 1, nacq: int = 30, nburst: int = 9, lines: int = 13500,
         orbit = orbit0.shifted(off)
         row = []
         for k, ((fl, nl), tc) in enumerate(zip(windows, tcs)):
-            L = int(rng.integers(1350, 1551))
-            S = int(rng.integers(18000, 25001))
+            L = int(rng.integers(pulses[0], pulses[1] + 1))
+            S = int(rng.integers(samples[0], samples[1] + 1))
             row.append(stack_burst(grid, orbit, fl, nl, L, S, tc, centre_range,
                                    seed=seed * 100000 + 100 * a + k,
-                                   target_seed=seed * 1000 + k))
+                                   target_seed=seed * 1000 + k, aperture=aperture,
+                                   ntargets=ntargets))
         bursts.append(row)
     return Stack(grid=grid, bursts=bursts, baselines=np.array(baselines))

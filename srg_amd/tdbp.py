@@ -124,6 +124,7 @@ def _sig(lib):
         "tdbp_batch_add": (C.c_int, [_VP, _GP, _SP, _VP, _VP, _VP]),
         "tdbp_batch_run": (C.c_int, [_VP, _VP]),
         "tdbp_batch_set_merge": (C.c_int, [_VP, C.c_int, _VP, C.c_int, C.c_int, C.c_int]),
+        "tdbp_dropin_timing": (C.c_int, [C.POINTER(C.c_double), C.c_int]),
         "tdbp_compare": (C.c_int, [_VP, _VP, C.c_size_t, C.POINTER(C.c_double), _VP]),
         "tdbp_merge_geo": (C.c_int, [_VP, C.c_int, C.c_int, C.c_int, C.c_int, _VP, _VP]),
         "tdbp_batch_pairs": (C.c_int64, [_VP]),
@@ -158,7 +159,7 @@ def exported_symbols():
         "tdbp_pair_count", "tdbp_pack_samples", "tdbp_pixelint", "tdbp_backproject",
         "tdbp_backproject_variant", "tdbp_autotune", "tdbp_set_variant",
         "tdbp_variant_count", "tdbp_variant_info", "tdbp_batch_create", "tdbp_batch_add",
-        "tdbp_batch_run", "tdbp_batch_set_merge", "tdbp_merge_geo", "tdbp_compare", "tdbp_batch_pairs",
+        "tdbp_batch_run", "tdbp_batch_set_merge", "tdbp_merge_geo", "tdbp_compare", "tdbp_dropin_timing", "tdbp_batch_pairs",
         "tdbp_batch_variant",
         "tdbp_batch_device_bytes", "tdbp_batch_destroy", "tdbp_synth_clutter",
         "tdbp_synth_targets", "azimuth_compress_",

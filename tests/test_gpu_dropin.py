@@ -1,0 +1,156 @@
+"""GPU tests of the drop-in entry azimuth_compress_ called as the Fortran
+driver calls it, on files: two overlapping bursts of one scene written into
+one .geo in order (the later burst owns the overlap) against the oracle's
+restatement of the reference GPU entry; mixed precision selected by
+TDBP_PRECISION against the FP64 entry; .geo rows past the end of the file;
+a short DEM file leaves the .geo untouched and reports the error."""
+import os
+import tempfile
+
+import numpy as np
+import pytest
+
+from oracle import pyoracle
+from srg_amd import dropin, synth
+from srg_amd import tdbp as T
+
+pytestmark = pytest.mark.gpu
+
+
+def errors(g, o, mask):
+    g = g[mask].astype(np.complex128)
+    o = o[mask].astype(np.complex128)
+    w = np.abs(o) ** 2
+    ph = float(np.sqrt(np.sum(w * np.angle(g * np.conj(o)) ** 2) / np.sum(w)))
+    amp = float(np.sqrt(np.sum((np.abs(g) - np.abs(o)) ** 2) / np.sum(w)))
+    return ph, amp
+
+
+@pytest.fixture(scope="module")
+def scene():
+    stk = synth.ts_stack(nacq=1, nburst=2, lines=240, W=160, pulses=(600, 600),
+                         samples=(5000, 5000), rows=(130, 140), aperture=256, seed=3)
+    bursts = stk.bursts[0]
+    data = [synth.host_samples(sc) for sc in bursts]
+    return stk, bursts, data
+
+
+def calls(stk, bursts, data):
+    return [dropin.BurstCall.from_burst(d, sc.geom, sc.satloc, stk.grid.lines,
+                                        stk.grid.firstlat, sc.latlons)
+            for sc, d in zip(bursts, data)]
+
+
+def run_files(td, name, stk, geo0, fn):
+    demf, geof = os.path.join(td, name + ".dem"), os.path.join(td, name + ".geo")
+    stk.grid.dem.astype(np.int16).tofile(demf)
+    geo0.tofile(geof)
+    fdd, fdo = os.open(demf, os.O_RDONLY), os.open(geof, os.O_RDWR)
+    try:
+        fn(fdo, fdd)
+    finally:
+        os.close(fdd)
+        os.close(fdo)
+    return np.fromfile(geof, np.complex64)
+
+
+def test_dropin_two_bursts_in_order_match_oracle(gpu, oracle, scene):
+    stk, bursts, data = scene
+    g0 = bursts[0].geom
+    assert bursts[1].geom.firstline < g0.firstline + g0.nlines   # the windows overlap
+    shape = (stk.grid.lines, stk.grid.W)
+    geo0 = (np.random.default_rng(1).standard_normal(shape) * 1e-3).astype(np.complex64)
+    cs = calls(stk, bursts, data)
+
+    def ours(fdo, fdd):
+        for c in cs:
+            c.call(fdo, fdd)
+
+    def ref(fdo, fdd):
+        for sc, d in zip(bursts, data):
+            g = sc.geom
+            oracle.azimuth_compress(d, sc.satloc, g.L, g.S, g.demwidth, stk.grid.lines, fdo,
+                                    fdd, g.deltalat, g.deltalon, stk.grid.firstlat,
+                                    g.firstlon, sc.latlons, g.timeorbit, g.xx, g.vv,
+                                    g.rngstart, g.rngend, g.tmid, g.xyz_mid, g.vel_mid,
+                                    g.dmrg, g.wvl, g.A, g.angc0, g.angc1, g.prf,
+                                    semantics=pyoracle.GPU_SEMANTICS)
+
+    with tempfile.TemporaryDirectory() as td:
+        a = run_files(td, "gpu", stk, geo0, ours).reshape(shape)
+        b = run_files(td, "orc", stk, geo0, ref).reshape(shape)
+    changed = b != geo0
+    assert changed.sum() > 0.5 * changed.size
+    np.testing.assert_array_equal(a != geo0, changed)
+    np.testing.assert_array_equal(a[~changed].view(np.uint32), geo0[~changed].view(np.uint32))
+    ph, amp = errors(a, b, changed)
+    assert ph <= 1e-6 and amp <= 1e-6
+    # the overlap rows hold the second burst's values
+    g1 = bursts[1].geom
+    ov = slice(g1.firstline, g0.firstline + g0.nlines)
+    assert np.any(changed[ov])
+
+
+def test_dropin_mixed_precision_env(gpu, scene):
+    stk, bursts, data = scene
+    shape = (stk.grid.lines, stk.grid.W)
+    geo0 = np.zeros(shape, np.complex64)
+    cs = calls(stk, bursts, data)
+    out = {}
+    with tempfile.TemporaryDirectory() as td:
+        for prec in ("fp64", "mixed_fp16", "mixed_cf32"):
+            os.environ["TDBP_PRECISION"] = prec
+            try:
+                out[prec] = run_files(td, prec, stk, geo0,
+                                      lambda fo, fd: [c.call(fo, fd) for c in cs]).reshape(shape)
+            finally:
+                os.environ.pop("TDBP_PRECISION", None)
+            t = dropin.last_timing()
+            assert t["total"] > 0 and t["compute"] > 0
+    m = out["fp64"] != 0
+    assert m.sum() > 0.5 * m.size
+    for prec in ("mixed_fp16", "mixed_cf32"):
+        np.testing.assert_array_equal(out[prec] != 0, m)
+        ph, amp = errors(out[prec], out["fp64"], m)
+        assert ph <= 1e-3 and amp <= 2e-4, (prec, ph, amp)
+
+
+def test_dropin_geo_rows_past_end_of_file(gpu, scene):
+    stk, bursts, data = scene
+    W = stk.grid.W
+    g1 = bursts[1].geom
+    short = np.full((g1.firstline + 10, W), 5 + 5j, np.complex64)   # ends inside burst 1
+    c = calls(stk, bursts, data)[1]
+    with tempfile.TemporaryDirectory() as td:
+        full = run_files(td, "full", stk, np.zeros((stk.grid.lines, W), np.complex64),
+                         lambda fo, fd: c.call(fo, fd)).reshape(-1, W)
+        got = run_files(td, "short", stk, short, lambda fo, fd: c.call(fo, fd)).reshape(-1, W)
+    end = g1.firstline + g1.nlines
+    assert got.shape[0] == end
+    win = full[g1.firstline:end]
+    gw = got[g1.firstline:end]
+    m = win != 0
+    np.testing.assert_array_equal(gw[m].view(np.uint32), win[m].view(np.uint32))
+    old = np.zeros_like(gw)
+    old[:10] = 5 + 5j                                                # the file's own rows
+    np.testing.assert_array_equal(gw[~m], old[~m])
+    assert np.all(got[:g1.firstline] == 5 + 5j)
+
+
+def test_dropin_short_dem_is_an_error(gpu, scene):
+    stk, bursts, data = scene
+    c = calls(stk, bursts, data)[1]
+    geo0 = np.full((stk.grid.lines, stk.grid.W), 1 + 2j, np.complex64)
+    with tempfile.TemporaryDirectory() as td:
+        demf, geof = os.path.join(td, "s.dem"), os.path.join(td, "s.geo")
+        stk.grid.dem[:200].astype(np.int16).tofile(demf)        # window needs 240 rows
+        geo0.tofile(geof)
+        fdd, fdo = os.open(demf, os.O_RDONLY), os.open(geof, os.O_RDWR)
+        try:
+            with pytest.raises(T.TdbpError, match="DEM read"):
+                c.call(fdo, fdd)
+        finally:
+            os.close(fdd)
+            os.close(fdo)
+        after = np.fromfile(geof, np.complex64)
+    assert np.array_equal(after.view(np.uint32), geo0.ravel().view(np.uint32))

@@ -19,7 +19,7 @@ import numpy as np
 import pytest
 
 from oracle import pyoracle
-from srg_amd import synth
+from srg_amd import dropin, synth
 from srg_amd import tdbp as T
 
 pytestmark = pytest.mark.gpu
@@ -238,7 +238,6 @@ def test_dropin_matches_oracle_gpu_semantics(gpu, oracle):
             fdd, fdo = os.open(demf, os.O_RDWR), os.open(geof, os.O_RDWR)
             if who == "gpu":
                 _call_dropin(data, sc, full.shape[0], fdo, fdd, fl, latlons)
-                assert T.lib().tdbp_dropin_status() == 0, T.lib().tdbp_last_error()
             else:
                 oracle.azimuth_compress(data, sc.satloc, g.L, g.S, g.demwidth, full.shape[0],
                                         fdo, fdd, g.deltalat, g.deltalon, fl, g.firstlon,
@@ -259,31 +258,8 @@ def test_dropin_matches_oracle_gpu_semantics(gpu, oracle):
 
 
 def _call_dropin(data, sc, demlength, fdout, fddem, firstlat, latlons):
-    import ctypes as C
-
-    g = sc.geom
-    sat = np.zeros((100000, 3))
-    sat[:g.L] = sc.satloc
-    tt = np.zeros(1000)
-    tt[:len(g.timeorbit)] = g.timeorbit
-    xx = np.zeros((1000, 3))
-    xx[:len(g.timeorbit)] = g.xx
-    vv = np.zeros((1000, 3))
-    vv[:len(g.timeorbit)] = g.vv
-    ll = np.ascontiguousarray(latlons, np.float64)
-    xm = np.ascontiguousarray(g.xyz_mid, np.float64)
-    vm = np.ascontiguousarray(g.vel_mid, np.float64)
-    tp = np.zeros(100000)
-    b = np.ascontiguousarray(data, np.complex64)
-    i = lambda v: C.byref(C.c_int(int(v)))  # noqa: E731
-    d = lambda v: C.byref(C.c_double(float(v)))  # noqa: E731
-    dp = lambda a: a.ctypes.data_as(C.POINTER(C.c_double))  # noqa: E731
-    T.lib().azimuth_compress_(
-        b.ctypes.data, dp(sat), i(g.L), i(g.S), i(g.demwidth), i(demlength), i(fdout),
-        i(fddem), d(g.deltalat), d(g.deltalon), d(firstlat), d(g.firstlon), dp(ll), dp(tt),
-        dp(xx), dp(vv), i(len(g.timeorbit)), d(g.rngstart), d(g.rngend), d(0), d(0),
-        d(g.tmid), dp(xm), dp(vm), dp(tp), d(1 / g.prf), d(g.dmrg), d(g.wvl), i(g.A), i(0),
-        d(g.angc0), d(g.angc1), d(g.prf))
+    dropin.BurstCall.from_burst(data, sc.geom, sc.satloc, demlength, firstlat,
+                                latlons).call(fdout, fddem)
 
 
 def test_pack_roundtrip(gpu):

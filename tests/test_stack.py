@@ -135,3 +135,13 @@ def test_stack_baselines_perpendicular(stack):
         sc = stack.bursts[a][4]
         p = orbit.pos(sc.times) + stack.baselines[a]
         assert np.max(np.abs(sc.satloc - p)) < 1e-3
+
+
+def test_iw_scene_shape():
+    scenes = synth.iw_scene()
+    assert len(scenes) == 27
+    for sc in scenes:
+        g = sc.geom
+        assert 1350 <= g.L <= 1550 and 18000 <= g.S <= 25000
+        assert 1000 <= g.demwidth <= 4500 and 1200 <= g.nlines <= 1600
+        assert g.A < g.L and len(sc.targets_px) == 3
