@@ -244,6 +244,84 @@ def dropin_bench(args, synth, T):
     print(json.dumps(line), flush=True)
 
 
+def rc_bench(args, synth, T):
+    """Range compression of one IW burst on the device (1500 decoded lines of
+    30000 samples, IW1 chirp of 3372 samples, 24400 echo samples -> 21028
+    compressed), fused with the cf16 packing, then the mixed backprojection
+    of the packed burst onto 4000 x 1500 px: raw echoes to image."""
+    from srg_amd import raw
+
+    h = raw.header_bytes(raw.IW1)
+    c = T.chirp_decode(h)
+    S = c.nvalid + 10
+    src = synth.fbm_surface((2048, 2048), seed=7)
+    sc = synth.make_burst(1500, S, 4000, 1500, dem="fbm", dem_source=src, ntargets=5,
+                          clutter_sigma=0.0, seed=0)
+    g = sc.geom
+    d_raw = T.DeviceArray((g.L, raw.RAW_STRIDE), np.complex64)
+    raw.device_lines(sc, c, h, d_raw, noise=0.1, seed=3)
+    pw = T.DeviceArray((g.L, S), np.uint32)
+    ps = T.DeviceArray((g.L,), np.float32)
+    cf = T.DeviceArray((g.L, S), np.complex64)
+    d_sat = T.DeviceArray.from_host(sc.satloc)
+    d_dem = T.DeviceArray.from_host(sc.dem)
+    d_coef = T.DeviceArray((g.nlines, 3), np.float64)
+    d_pairs = T.DeviceArray((g.nlines,), np.int64)
+    out = T.DeviceArray((g.nlines, g.demwidth), np.complex64)
+    T.fit_aperture(g, d_dem, d_coef)
+    T.pair_count(g, d_coef, d_pairs)
+    pairs = int(d_pairs.download().sum())
+    stream = T.Stream()
+    ev0, ev1 = T.Event(), T.Event()
+
+    def timed(fn, reps):
+        fn()
+        stream.sync()
+        ev0.record(stream)
+        for _ in range(reps):
+            fn()
+        ev1.record(stream)
+        stream.sync()
+        return ev0.elapsed_ms(ev1) / reps
+
+    packed = lambda: T.range_compress(d_raw, c, None, packed=(T.CF16, pw, ps),  # noqa: E731
+                                      stream=stream)
+    both = lambda: T.range_compress(d_raw, c, cf, packed=(T.CF16, pw, ps),  # noqa: E731
+                                    stream=stream)
+    tdbp_ = lambda: T.backproject(g, T.samples(pw, T.CF16, ps), d_sat, d_dem, out,  # noqa: E731
+                                  T.MIXED, d_coef=d_coef, stream=stream)
+    ms_rc = timed(packed, args.steps)
+    ms_both = timed(both, max(2, args.steps // 2))
+    ms_bp = timed(tdbp_, args.steps)
+    ms_e2e = timed(lambda: (packed(), tdbp_()), args.steps)
+    alg = g.L * ((c.nsamps + 10) * 8 + S * 4 + 4)     # read echoes, write packed + scale
+    gbps = alg / (ms_rc * 1e-3) / 1e9
+    img = out.download()
+    foc = [abs(np.angle(img[r, col] * np.conj(a))) for (r, col), a in
+           zip(sc.targets_px, sc.targets_amp)]
+    log(f"range compression (fused cf16): {ms_rc:.3f} ms/burst ({g.L / ms_rc * 1e3:.0f} lines/s, "
+        f"{gbps:.0f} GB/s algorithmic); with cf32 copy {ms_both:.3f} ms; backprojection "
+        f"{ms_bp:.3f} ms; raw -> image {ms_e2e:.3f} ms; target phase errors (rad) "
+        f"{[round(float(x), 4) for x in foc]}")
+    line = {"metric": "range-compressed lines/s (32768-point matched filter, fused cf16 pack)",
+            "value": round(g.L / ms_rc * 1e3, 1), "unit": "lines/s", "n_gpus": 1,
+            "steps": args.steps, "warmup": 1, "ms_per_step": round(ms_rc, 4),
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+            "dtype": "f32", "data": "synthetic (GPU-generated chirp echoes of 5 targets + noise)",
+            "config": {"workload": f"one IW burst: {g.L} decoded lines x 30000 samples, IW1 chirp "
+                                   f"{c.npts} samples, {c.nsamps} echo -> {c.nvalid} compressed; "
+                                   f"then mixed backprojection onto 4000 x 1500 px",
+                       "pairs": pairs},
+            "roofline": {"bound": "hbm", "achieved": round(gbps, 1), "peak": 8000.0,
+                         "unit": "GB/s", "frac": round(gbps / 8000.0, 4), "traffic": None,
+                         "algorithmic_bytes_per_burst": alg},
+            "ms_with_cf32_copy": round(ms_both, 4), "ms_backprojection": round(ms_bp, 4),
+            "ms_raw_to_image": round(ms_e2e, 4),
+            "raw_to_image_gpps": round(pairs / ms_e2e / 1e6, 2),
+            "target_phase_err_rad": [round(float(x), 5) for x in foc]}
+    print(json.dumps(line), flush=True)
+
+
 def stack_bench(args, synth, T):
     """Time-series stack: 30 acquisitions x 9 uneven bursts onto one common
     13500 x 4000 grid, cf16 mixed precision, admitted into HBM-sized batches,
@@ -357,12 +435,13 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--plain-passes", type=int, default=2)
     ap.add_argument("--workload", default="scene",
-                    choices=["scene", "burst", "small", "stack", "dropin"],
+                    choices=["scene", "burst", "small", "stack", "dropin", "rc"],
                     help="scene: batched 27-burst IW scene (the headline); burst: one full "
                          "IW burst, plain vs mixed per storage type; small: the 512 x 4096 "
                          "-> 256 x 256 burst per storage type against the FP64 oracle; "
                          "stack: 30 acquisitions x 9 bursts onto one 13500 x 4000 grid; "
-                         "dropin: azimuth_compress_ end to end on one IW burst")
+                         "dropin: azimuth_compress_ end to end on one IW burst; rc: range "
+                         "compression of one IW burst and raw -> image")
     ap.add_argument("--acqs", type=int, default=30, help="stack: acquisitions")
     ap.add_argument("--hbm-budget", type=float, default=0.0,
                     help="stack: admission budget in GiB (default: free HBM less a reserve)")
@@ -387,6 +466,8 @@ def main():
         return stack_bench(args, synth, T)
     if args.workload == "dropin":
         return dropin_bench(args, synth, T)
+    if args.workload == "rc":
+        return rc_bench(args, synth, T)
     t0 = time.time()
     nsub = 3 if args.bursts >= 3 else 1
     scenes = synth.iw_scene(seed=1, nsub=nsub, nburst=max(1, args.bursts // nsub))

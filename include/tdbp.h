@@ -274,6 +274,66 @@ int tdbp_merge_geo(void *d_grid, int W, int grid_lines, int firstline, int nline
 int tdbp_compare(const void *d_img, const void *d_ref, size_t n, double *sums,
                  void *stream);
 
+/* ------------------------------------------------- range compression */
+
+/* Chirp and echo-window parameters of a swath, decoded from the 80-byte
+ * packet header of its first decoded line (sentinel/processsub.f90:75-86;
+ * big-endian header fields as in2/in3 of :163-173; sampling rate table of
+ * :181-212).  Stored in FP32 as the reference stores them. */
+typedef struct tdbp_chirp {
+  int32_t rangedecimation; /* header byte 40 */
+  int32_t nsamps;          /* echo samples per line, 2 x bytes 65-66 */
+  int32_t npts;            /* chirp samples, int(pulselength * samplefreq) */
+  int32_t nvalid;          /* nsamps - npts: compressed samples kept */
+  float samplefreq;        /* MHz */
+  float ramprate;          /* MHz/us, bytes 42-43 (sign bit + 15 bits) */
+  float startfreq;         /* MHz, bytes 44-45 (sign bit + 15 bits) */
+  float pulselength;       /* us, bytes 46-48 */
+} tdbp_chirp;
+
+int tdbp_chirp_decode(const uint8_t *header80, tdbp_chirp *c);
+
+/* Range compression of nlines decoded echo lines on the device, replacing
+ * the per-line FFTW loop of sentinel/processsub.f90:96-126.  d_raw: cf32
+ * [nlines][raw_stride], each line's first 10 samples the packet header
+ * (raw_stride 30000 in the reference), nsamps echo samples after it.
+ * Outputs (either or both):
+ *   d_out     cf32 [nlines][out_stride >= nvalid + 10]: the header samples,
+ *             then nvalid compressed samples (the reference rangeprocdata
+ *             layout, :121-122);
+ *   d_packed  CF16/CBF16 words [nlines][nvalid + 10] and d_scale [nlines]:
+ *             the backprojection input (header samples zero, as
+ *             sentinel/backprojectgpusub.f90:190 leaves them) in the storage
+ *             of tdbp_pack_samples.
+ * The transform is the reference's: 32768-point forward FFT of the
+ * zero-padded echo, times conj(FFT(reference chirp / npts)), inverse FFT
+ * without normalisation, in FP32. */
+int tdbp_range_compress(const void *d_raw, int nlines, int raw_stride,
+                        const tdbp_chirp *c, void *d_out, int out_stride,
+                        void *d_packed, float *d_scale, int dtype, void *stream);
+
+/* ------------------------------------------ interferograms (.geo stacks) */
+
+/* int/crossmul.f90 on device images: d_img1, d_img2 cf32 [>= nvalid][na]
+ * (the same pointer for an image with itself); uses the first
+ * (nvalid / looksdn) * looksdn rows.  Outputs cf32 [nvalid / looksdn]
+ * [na / looksac]: d_igram = looked sum of up1 conj(up2) of the two-times
+ * upsampled rows, d_amp = (sqrt(sum |up1|^2), sqrt(sum |up2|^2)); rows
+ * scaled by `scale` before upsampling; mask != 0 zeroes outputs whose
+ * unscaled input sum is <= 1e-6 in either image (crossmul.f90:110-122, the
+ * reference default).  Upsampling: zero-pad to nnn = next power of two >= na,
+ * forward FFT, spectrum halves to both ends of 2 nnn, inverse FFT, / nnn / 2
+ * (:127-148).  looksdn <= 512. */
+int tdbp_crossmul(const void *d_img1, const void *d_img2, int na, int nvalid, float scale,
+                  int looksac, int looksdn, int mask, void *d_igram, void *d_amp,
+                  void *stream);
+
+/* int/makecc.f90 on device arrays: d_igram, d_amp cf32 [nlines][npix] ->
+ * d_out f32 [nlines][2 npix]: per line the amplitude row re(amp) im(amp),
+ * then the correlation row |igram| / amplitude (0 where amplitude <= 0). */
+int tdbp_makecc(const void *d_igram, const void *d_amp, int npix, int nlines, float *d_out,
+                void *stream);
+
 /* ------------------------------------------- synthetic data (bench/tests) */
 
 /* Band-limited complex Gaussian clutter: white CN(0, sigma^2) from a
@@ -287,6 +347,16 @@ int tdbp_synth_clutter(void *d_cf32, int L, int S, uint64_t seed, float sigma,
  * a * sinc((r - r_k(p)) / res) * exp(-j 4 pi R_k(p) / wvl) on the samples
  * r within +-halfwidth of r_k(p) = (R_k(p) - rngstart) / dmrg.  Host
  * arrays; d_satloc f64 [L][3]. */
+/* Decoded echo lines cf32 [L][stride]: samples 0-9 the 80 header bytes,
+ * then c->nsamps samples holding, per target t, the chirp of c delayed by
+ * delay[p*ntargets+t] samples (host array) with carrier
+ * exp(-j 2 pi carrier[p*ntargets+t]) and amplitude amp[2t] + j amp[2t+1],
+ * plus white circular Gaussian noise (std `noise`); zero after.  Host arrays
+ * are copied before the call returns (it synchronises the stream). */
+int tdbp_synth_raw(void *d_raw, int L, int stride, const tdbp_chirp *c,
+                   const uint8_t *header80, const double *delay, const double *carrier,
+                   const float *amp, int ntargets, float noise, uint64_t seed,
+                   void *stream);
 int tdbp_synth_targets(void *d_cf32, const tdbp_geom *g, const double *d_satloc,
                        const double *xyz, const float *amp, int ntargets,
                        float res, int halfwidth, void *stream);

@@ -547,6 +547,43 @@ int tdbp_synth_clutter(void* d, int L, int S, uint64_t seed, float sigma,
                    "synth_clutter launch");
 }
 
+int tdbp_synth_raw(void* d_raw, int L, int stride, const tdbp_chirp* c,
+                   const uint8_t* header80, const double* delay, const double* carrier,
+                   const float* amp, int ntargets, float noise, uint64_t seed, void* stream) {
+  if (!d_raw || !c || !header80 || L < 1 || L > 65535 || stride < 10 + c->nsamps ||
+      c->nsamps < 1 || c->npts < 1 || ntargets < 0 || (ntargets > 0 && (!delay || !carrier || !amp)))
+    return set_error(TDBP_ERR_ARG, "synth_raw: bad arguments");
+  int st = current_device_ready();
+  if (st) return st;
+  hipStream_t s = (hipStream_t)stream;
+  const size_t nd = (size_t)L * (ntargets > 0 ? ntargets : 1);
+  double *dd = nullptr, *dc = nullptr;
+  float2* da = nullptr;
+  if ((st = hip_check(hipMallocAsync((void**)&dd, nd * 8, s), "synth_raw scratch")) ||
+      (st = hip_check(hipMallocAsync((void**)&dc, nd * 8, s), "synth_raw scratch")) ||
+      (st = hip_check(hipMallocAsync((void**)&da, 8 * (ntargets > 0 ? ntargets : 1), s),
+                      "synth_raw scratch")))
+    return st;
+  if (ntargets > 0) {
+    if ((st = hip_check(hipMemcpyAsync(dd, delay, nd * 8, hipMemcpyHostToDevice, s),
+                        "synth_raw upload")) ||
+        (st = hip_check(hipMemcpyAsync(dc, carrier, nd * 8, hipMemcpyHostToDevice, s),
+                        "synth_raw upload")) ||
+        (st = hip_check(hipMemcpyAsync(da, amp, 8 * (size_t)ntargets, hipMemcpyHostToDevice, s),
+                        "synth_raw upload")))
+      return st;
+  }
+  st = hip_check(launch_synth_raw((float2*)d_raw, L, stride, c->nsamps, c->npts, c->samplefreq,
+                                  c->startfreq, c->ramprate, dd, dc, da, ntargets, noise, seed,
+                                  header80, s),
+                 "synth_raw launch");
+  (void)hipFreeAsync(dd, s);
+  (void)hipFreeAsync(dc, s);
+  (void)hipFreeAsync(da, s);
+  if (!st) st = hip_check(hipStreamSynchronize(s), "synth_raw sync");
+  return st;
+}
+
 int tdbp_synth_targets(void* d, const tdbp_geom* g, const double* d_satloc,
                        const double* xyz, const float* amp, int ntargets,
                        float res, int halfwidth, void* stream) {

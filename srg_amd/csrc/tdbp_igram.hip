@@ -1,0 +1,240 @@
+// tdbp_igram.hip -- interferogram formation on co-registered .geo images on
+// the device: the programs int/crossmul.f90 and int/makecc.f90.
+//
+// crossmul (int/crossmul.f90:86-199), per output line of looksdn input rows:
+//   * masks from the unscaled rows: per output column, the complex sum over
+//     the looksdn rows and looksac columns; |sum| <= 1e-6 marks no data
+//     (:110-122);
+//   * each row times `scale`, zero-padded to nnn = the next power of two >= na,
+//     forward FFT; the spectrum halves placed at both ends of a 2 nnn array;
+//     inverse FFT; divided by nnn, then by 2: the row upsampled by two
+//     (:127-148);
+//   * igram = up1 conj(up2), amp = (|up1|^2, |up2|^2) over the first 2 na
+//     samples; adjacent pairs summed back to na columns; summed over the
+//     looksdn rows, then over looksac columns; amp square-rooted per
+//     component; masked outputs zero (:150-184).
+// makecc (int/makecc.f90:46-60): amp = re(b) im(b), cc = |a| / amp (0 where
+// amp <= 0), written as [amp row | cc row].
+//
+// Rows go through the transforms in batches of whole output lines; the two
+// upsampled images of a batch are held at once and the cross product, looks
+// and masks are one kernel reading them.
+#include <hip/hip_runtime.h>
+
+#include <mutex>
+
+#include "tdbp_api_internal.h"
+#include "tdbp_fft.h"
+#include "tdbp_internal.h"
+
+using namespace tdbp;
+
+namespace {
+
+constexpr int kRows = 512;   // rows per transform batch (rounded down to whole looks)
+constexpr int kMaxDev = 64;
+
+__global__ void k_up_load(const float2* __restrict__ img, int na, int nrows, int row0,
+                          int nvalid_rows, int nnn, float scale, float2* __restrict__ buf) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = blockIdx.y;
+  if (j >= nnn) return;
+  const int r = row0 + i;
+  float2 v = make_float2(0.f, 0.f);
+  if (i < nrows && r < nvalid_rows && j < na) {
+    const float2 a = img[(size_t)r * na + j];
+    v = make_float2(a.x * scale, a.y * scale);
+  }
+  buf[(size_t)i * nnn + j] = v;
+}
+
+__global__ void k_up_spread(const float2* __restrict__ buf, int nnn, float2* __restrict__ up) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = blockIdx.y;
+  const int n2 = 2 * nnn;
+  if (k >= n2) return;
+  float2 v = make_float2(0.f, 0.f);
+  if (k < nnn / 2)
+    v = buf[(size_t)i * nnn + k];
+  else if (k >= n2 - nnn / 2)
+    v = buf[(size_t)i * nnn + (k - nnn)];
+  up[(size_t)i * n2 + k] = v;
+}
+
+__device__ __forceinline__ float2 up_at(const float2* up, size_t idx, float fn) {
+  const float2 a = up[idx];
+  return make_float2((a.x / fn) / 2.0f, (a.y / fn) / 2.0f);
+}
+
+__global__ void k_xmul(const float2* __restrict__ up1, const float2* __restrict__ up2, int nnn,
+                       const float2* __restrict__ img1, const float2* __restrict__ img2, int na,
+                       int row0, int nout_lines_chunk, int out_line0, int looksac, int looksdn,
+                       int mask, float2* __restrict__ igram, float2* __restrict__ amp) {
+  const int nw = na / looksac;
+  const int oc = blockIdx.x * blockDim.x + threadIdx.x;
+  const int ol = blockIdx.y;
+  if (oc >= nw || ol >= nout_lines_chunk) return;
+  const int n2 = 2 * nnn;
+  const float fn = (float)nnn;
+  float ig_r = 0.f, ig_i = 0.f, a1 = 0.f, a2 = 0.f;
+  float m1r = 0.f, m1i = 0.f, m2r = 0.f, m2i = 0.f;
+  for (int k = 0; k < looksac; ++k) {
+    const int j = oc * looksac + k;
+    float cr = 0.f, ci = 0.f, c1 = 0.f, c2 = 0.f;      // sums over the rows
+    float s1r = 0.f, s1i = 0.f, s2r = 0.f, s2i = 0.f;
+    for (int r = 0; r < looksdn; ++r) {
+      const int il = ol * looksdn + r;                   // row in the batch
+      const size_t b = (size_t)il * n2 + 2 * j;
+      const float2 ua = up_at(up1, b, fn), ub = up_at(up1, b + 1, fn);
+      const float2 va = up_at(up2, b, fn), vb = up_at(up2, b + 1, fn);
+      const float pr = (ua.x * va.x + ua.y * va.y) + (ub.x * vb.x + ub.y * vb.y);
+      const float pi = (ua.y * va.x - ua.x * va.y) + (ub.y * vb.x - ub.x * vb.y);
+      const float ha = hypotf(ua.x, ua.y), hb = hypotf(ub.x, ub.y);
+      const float ga = hypotf(va.x, va.y), gb = hypotf(vb.x, vb.y);
+      cr += pr;
+      ci += pi;
+      c1 += ha * ha + hb * hb;
+      c2 += ga * ga + gb * gb;
+      if (mask) {
+        const size_t o = (size_t)(row0 + il) * na + j;
+        const float2 x1 = img1[o], x2 = img2[o];
+        s1r += x1.x;
+        s1i += x1.y;
+        s2r += x2.x;
+        s2i += x2.y;
+      }
+    }
+    ig_r += cr;
+    ig_i += ci;
+    a1 += c1;
+    a2 += c2;
+    m1r += s1r;
+    m1i += s1i;
+    m2r += s2r;
+    m2i += s2i;
+  }
+  float2 g = make_float2(ig_r, ig_i);
+  float2 a = make_float2(sqrtf(a1), sqrtf(a2));
+  if (mask && (hypotf(m1r, m1i) <= 1.e-6f || hypotf(m2r, m2i) <= 1.e-6f)) {
+    g = make_float2(0.f, 0.f);
+    a = make_float2(0.f, 0.f);
+  }
+  const size_t o = (size_t)(out_line0 + ol) * nw + oc;
+  igram[o] = g;
+  amp[o] = a;
+}
+
+__global__ void k_makecc(const float2* __restrict__ ig, const float2* __restrict__ am, int npix,
+                         int nlines, float* __restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int l = blockIdx.y;
+  if (i >= npix || l >= nlines) return;
+  const float2 a = ig[(size_t)l * npix + i], b = am[(size_t)l * npix + i];
+  const float amp = b.x * b.y;
+  const float cc = amp > 0.f ? hypotf(a.x, a.y) / amp : 0.f;
+  out[(size_t)l * 2 * npix + i] = amp;
+  out[(size_t)l * 2 * npix + npix + i] = cc;
+}
+
+// Per device: the transform buffers of one batch.
+struct IgState {
+  size_t nnn = 0;
+  float2* buf = nullptr;   // [kRows][nnn]
+  float2* up1 = nullptr;   // [kRows][2 nnn]
+  float2* up2 = nullptr;
+};
+std::mutex g_ig_mu;
+IgState g_ig[kMaxDev];
+
+int buffers(IgState& s, size_t nnn) {
+  if (s.nnn >= nnn && s.buf) return TDBP_OK;
+  (void)hipDeviceSynchronize();
+  if (s.buf) (void)hipFree(s.buf);
+  if (s.up1) (void)hipFree(s.up1);
+  if (s.up2) (void)hipFree(s.up2);
+  s = IgState();
+  if (hipMalloc((void**)&s.buf, sizeof(float2) * kRows * nnn) != hipSuccess ||
+      hipMalloc((void**)&s.up1, sizeof(float2) * kRows * 2 * nnn) != hipSuccess ||
+      hipMalloc((void**)&s.up2, sizeof(float2) * kRows * 2 * nnn) != hipSuccess)
+    return set_error(TDBP_ERR_NOMEM, "interferogram buffers (nnn %zu)", nnn);
+  s.nnn = nnn;
+  return TDBP_OK;
+}
+
+int upsample(const float2* img, int na, int rows, int row0, int nvalid_rows, size_t nnn,
+             float scale, IgState& s, float2* up, hipStream_t st) {
+  hipLaunchKernelGGL(k_up_load, dim3((unsigned)((nnn + 255) / 256), kRows), dim3(256), 0, st,
+                     img, na, rows, row0, nvalid_rows, (int)nnn, scale, s.buf);
+  int e = hip_check(hipGetLastError(), "upsample load");
+  if (!e) e = fft_run(nnn, kRows, false, s.buf, st);
+  if (e) return e;
+  hipLaunchKernelGGL(k_up_spread, dim3((unsigned)((2 * nnn + 255) / 256), kRows), dim3(256), 0,
+                     st, s.buf, (int)nnn, up);
+  e = hip_check(hipGetLastError(), "upsample spread");
+  if (!e) e = fft_run(2 * nnn, kRows, true, up, st);
+  return e;
+}
+
+}  // namespace
+
+extern "C" {
+
+int tdbp_crossmul(const void* d_img1, const void* d_img2, int na, int nvalid, float scale,
+                  int looksac, int looksdn, int mask, void* d_igram, void* d_amp,
+                  void* stream) {
+  if (!d_img1 || !d_img2 || !d_igram || !d_amp || na < 1 || nvalid < 1 || looksac < 1 ||
+      looksdn < 1 || looksdn > kRows || na / looksac < 1 || nvalid / looksdn < 1)
+    return set_error(TDBP_ERR_ARG, "crossmul: width %d, %d lines, looks %d x %d", na, nvalid,
+                     looksac, looksdn);
+  if (na > (1 << 24)) return set_error(TDBP_ERR_LIMIT, "crossmul: width %d", na);
+  int st = current_device_ready();
+  if (st) return st;
+  size_t nnn = 2;
+  while (nnn < (size_t)na) nnn <<= 1;   // crossmul.f90:72-76
+  int dev = 0;
+  (void)hipGetDevice(&dev);
+  if (dev < 0 || dev >= kMaxDev) return set_error(TDBP_ERR_ARG, "device %d", dev);
+  std::lock_guard<std::mutex> lk(g_ig_mu);
+  IgState& s = g_ig[dev];
+  if ((st = buffers(s, nnn))) return st;
+  hipStream_t str = (hipStream_t)stream;
+  const float2* i1 = (const float2*)d_img1;
+  const float2* i2 = (const float2*)d_img2;
+  const int nout = nvalid / looksdn;             // output lines
+  const int lines_per_batch = kRows / looksdn;   // output lines per batch
+  const int used_rows = nout * looksdn;
+  for (int ol0 = 0; ol0 < nout; ol0 += lines_per_batch) {
+    const int nol = nout - ol0 < lines_per_batch ? nout - ol0 : lines_per_batch;
+    const int row0 = ol0 * looksdn;
+    const int rows = nol * looksdn;
+    if ((st = upsample(i1, na, rows, row0, used_rows, s.nnn, scale, s, s.up1, str))) return st;
+    if (i2 == i1) {
+      if ((st = hip_check(hipMemcpyAsync(s.up2, s.up1, sizeof(float2) * kRows * 2 * s.nnn,
+                                         hipMemcpyDeviceToDevice, str),
+                          "upsample copy")))
+        return st;
+    } else if ((st = upsample(i2, na, rows, row0, used_rows, s.nnn, scale, s, s.up2, str))) {
+      return st;
+    }
+    const int nw = na / looksac;
+    hipLaunchKernelGGL(k_xmul, dim3((unsigned)((nw + 127) / 128), nol), dim3(128), 0, str,
+                       s.up1, s.up2, (int)s.nnn, i1, i2, na, row0, nol, ol0, looksac, looksdn,
+                       mask ? 1 : 0, (float2*)d_igram, (float2*)d_amp);
+    if ((st = hip_check(hipGetLastError(), "crossmul launch"))) return st;
+  }
+  return TDBP_OK;
+}
+
+int tdbp_makecc(const void* d_igram, const void* d_amp, int npix, int nlines, float* d_out,
+                void* stream) {
+  if (!d_igram || !d_amp || !d_out || npix < 1 || nlines < 1 || nlines > 65535)
+    return set_error(TDBP_ERR_ARG, "makecc: %d x %d", npix, nlines);
+  int st = current_device_ready();
+  if (st) return st;
+  hipLaunchKernelGGL(k_makecc, dim3((unsigned)((npix + 255) / 256), nlines), dim3(256), 0,
+                     (hipStream_t)stream, (const float2*)d_igram, (const float2*)d_amp, npix,
+                     nlines, d_out);
+  return hip_check(hipGetLastError(), "makecc launch");
+}
+
+}  // extern "C"

@@ -80,6 +80,10 @@ hipError_t launch_pack(const float2* in, int L, int S, int dtype, void* out,
 hipError_t launch_synth_clutter(float2* d, int L, int S, uint64_t seed,
                                 float sigma, const float* taps, int ntaps,
                                 hipStream_t s);
+hipError_t launch_synth_raw(float2* d, int L, int stride, int nsamps, int npts, float sf,
+                            float f0, float ramp, const double* delay, const double* carrier,
+                            const float2* amp, int ntargets, float noise, uint64_t seed,
+                            const void* header80, hipStream_t s);
 hipError_t launch_synth_target(float2* d, const BurstDev& b, double tx,
                                double ty, double tz, float ar, float ai,
                                float res, int halfwidth, hipStream_t s);

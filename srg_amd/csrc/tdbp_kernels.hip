@@ -27,6 +27,7 @@
 
 #include "tdbp_geom.h"
 #include "tdbp_internal.h"
+#include "tdbp_pack.h"
 
 // Unroll of the test-free core pulse loop.
 #ifndef TDBP_CORE_UNROLL
@@ -756,28 +757,13 @@ __global__ void k_pack(const float2* __restrict__ in, int S, int dtype,
   __syncthreads();
   float mx = part[0];
   for (int w = 1; w < kBlock / kWave; ++w) mx = fmaxf(mx, part[w]);
-  const float sc = mx > 0.f ? mx * (1.0f / 16384.0f) : 1.0f;
+  const float sc = pulse_scale(mx);
   const float inv = 1.0f / sc;
   if (threadIdx.x == 0) scale[p] = sc;
   uint32_t* orow = out + (size_t)p * S;
   for (int i = threadIdx.x; i < S; i += blockDim.x) {
     const float2 v = row[i];
-    const float re = v.x * inv, im = v.y * inv;
-    uint32_t w;
-    if (dtype == TDBP_CF16) {
-      typedef _Float16 h2 __attribute__((ext_vector_type(2)));
-      h2 h;
-      h.x = (_Float16)re;
-      h.y = (_Float16)im;
-      __builtin_memcpy(&w, &h, 4);
-    } else {
-      // round to nearest even into the upper 16 bits
-      uint32_t a = __float_as_uint(re), c = __float_as_uint(im);
-      a = (a + 0x7fffu + ((a >> 16) & 1u)) >> 16;
-      c = (c + 0x7fffu + ((c >> 16) & 1u)) >> 16;
-      w = (a & 0xffffu) | (c << 16);
-    }
-    orow[i] = w;
+    orow[i] = pack_word(v.x * inv, v.y * inv, dtype);
   }
 }
 
@@ -796,6 +782,51 @@ __device__ __forceinline__ float2 gauss2(uint64_t seed, uint64_t k) {
   float s, c;
   sincospif(2.0f * u2, &s, &c);
   return make_float2(r * c, r * s);
+}
+
+// Decoded echo lines: the packet header in samples [0, 10), then nsamps echo
+// samples: per target, the transmitted chirp delayed by delay[p][t] samples
+// with carrier exp(-j 2 pi carrier[p][t]) and amplitude amp[t], plus white
+// circular Gaussian noise of standard deviation `noise`; zero beyond nsamps.
+struct RawHeader {
+  float2 w[10];
+};
+
+__global__ void k_synth_raw(float2* __restrict__ d, int L, int stride, int nsamps, int npts,
+                            float sf, float f0, float ramp, const double* __restrict__ delay,
+                            const double* __restrict__ carrier, const float2* __restrict__ amp,
+                            int ntargets, float noise, uint64_t seed, RawHeader hdr) {
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  const int p = blockIdx.y;
+  if (n >= stride || p >= L) return;
+  float2 v = make_float2(0.f, 0.f);
+  if (n < 10) {
+    v = hdr.w[n];
+  } else if (n - 10 < nsamps) {
+    const int k = n - 10;
+    double re = 0.0, im = 0.0;
+    for (int t = 0; t < ntargets; ++t) {
+      const double u = (double)k - delay[(size_t)p * ntargets + t];
+      if (u >= 0.0 && u < (double)npts) {
+        const double tt = u / (double)sf;
+        const double ph = 2.0 * 3.141592653589793 * ((double)f0 * tt + 0.5 * (double)ramp * tt * tt -
+                                                    carrier[(size_t)p * ntargets + t]);
+        double sn, cs;
+        sincos(ph, &sn, &cs);
+        const float2 a = amp[t];
+        re += a.x * cs - a.y * sn;
+        im += a.x * sn + a.y * cs;
+      }
+    }
+    if (noise > 0.f) {
+      const float2 g = gauss2(seed, (uint64_t)p * (uint64_t)stride + (uint64_t)n);
+      const float s = noise * 0.70710678f;
+      re += s * g.x;
+      im += s * g.y;
+    }
+    v = make_float2((float)re, (float)im);
+  }
+  d[(size_t)p * stride + n] = v;
 }
 
 struct Taps {
@@ -1084,6 +1115,18 @@ hipError_t launch_synth_clutter(float2* d, int L, int S, uint64_t seed,
   const size_t n = (size_t)L * S;
   hipLaunchKernelGGL(k_synth_clutter, dim3(blocks_for(n, kBlock)), dim3(kBlock),
                      0, s, d, L, S, seed, sigma, t, ntaps);
+  return hipGetLastError();
+}
+
+hipError_t launch_synth_raw(float2* d, int L, int stride, int nsamps, int npts, float sf,
+                            float f0, float ramp, const double* delay, const double* carrier,
+                            const float2* amp, int ntargets, float noise, uint64_t seed,
+                            const void* header80, hipStream_t s) {
+  RawHeader h;
+  __builtin_memcpy(&h, header80, sizeof h);
+  hipLaunchKernelGGL(k_synth_raw, dim3(blocks_for((size_t)stride, kBlock), L), dim3(kBlock), 0,
+                     s, d, L, stride, nsamps, npts, sf, f0, ramp, delay, carrier, amp,
+                     ntargets, noise, seed, h);
   return hipGetLastError();
 }
 

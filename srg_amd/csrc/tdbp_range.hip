@@ -1,0 +1,267 @@
+// tdbp_range.hip -- range compression of decoded echo lines on the device,
+// the matched filter of sentinel/processsub.f90:75-126:
+//   * chirp parameters from the 80-byte packet header of the first line
+//     (:75-86, with the header field readers in2/in3 and the sampling-rate
+//     table samplefrequency of :163-212, in the reference's FP32 storage);
+//   * reference function reftime(j) = exp(j phase(t_j)) / npts, t_j =
+//     j / samplefreq, phase = 2 pi startfreq t + pi ramprate t^2, over the
+//     npts chirp samples, transformed forward (:96-103);
+//   * per line: the nsamps echo samples after the header, zero-padded to
+//     32768, forward FFT, times conj(reference spectrum), inverse FFT
+//     without normalisation; the first nvalid = nsamps - npts samples are the
+//     compressed line (:108-123).
+// The transforms are cached rocFFT plans over batches of lines; loading, the
+// spectrum product and the output stage are kernels of this file.  The
+// output stage writes the reference layout (header samples, then the
+// compressed samples, cf32) and/or the backprojection input directly: the
+// header samples zeroed (sentinel/backprojectgpusub.f90:190) and the line
+// packed to FP16/BF16 with its per-pulse scale, the same words
+// tdbp_pack_samples makes.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <mutex>
+
+#include "tdbp_api_internal.h"
+#include "tdbp_fft.h"
+#include "tdbp_internal.h"
+#include "tdbp_pack.h"
+
+using namespace tdbp;
+
+namespace {
+
+constexpr int kFftLen = 32768;   // processsub.f90:39
+constexpr int kChunk = 256;      // lines per batched transform
+constexpr int kHdr = 10;         // header samples (80 bytes) per line
+constexpr double kFref = 37.53472224;
+constexpr int kMaxDev = 64;
+
+// ------------------------------------------------------------ header fields
+
+int be16(const uint8_t* b) { return (int)b[1] + 256 * (int)b[0]; }
+int be24(const uint8_t* b) { return (int)b[2] + 256 * (int)b[1] + 65536 * (int)b[0]; }
+
+// Sampling rate (MHz) per range decimation code, processsub.f90:181-212: the
+// ratio is a single-precision quotient times 4 (Fortran default reals), then
+// scaled by fref in double.  Codes outside the table: 0.
+double sample_frequency(int code) {
+  float r;
+  switch (code) {
+    case 0: r = 3.f / 4.f; break;
+    case 1: r = 2.f / 3.f; break;
+    case 3: r = 5.f / 9.f; break;
+    case 4: r = 4.f / 9.f; break;
+    case 5: r = 3.f / 8.f; break;
+    case 6: r = 1.f / 3.f; break;
+    case 7: r = 1.f / 6.f; break;
+    case 8: r = 3.f / 7.f; break;
+    case 9: r = 5.f / 16.f; break;
+    case 10: r = 3.f / 26.f; break;
+    case 11: r = 4.f / 11.f; break;
+    default: return 0.0;
+  }
+  const float r4 = r * 4.f;
+  return (double)r4 * kFref;
+}
+
+// ------------------------------------------------------------ kernels
+
+// Reference function (processsub.f90:96-101): t in FP32 (integer over real*4),
+// phase in FP64, cos/sin rounded to FP32, divided by npts in FP32.
+__global__ void k_chirp(float2* __restrict__ ref, int npts, float samplefreq,
+                        float startfreq, float ramprate) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= kFftLen) return;
+  float2 v = make_float2(0.f, 0.f);
+  if (j < npts) {
+#pragma clang fp contract(off)
+    const double pi = 3.141592653589793115997963468544185161590576171875;
+    const double t = (double)((float)j / samplefreq);
+    const double phase = 2.0 * pi * (double)startfreq * t + pi * (double)ramprate * t * t;
+    const float fn = (float)npts;
+    v = make_float2((float)cos(phase) / fn, (float)sin(phase) / fn);
+  }
+  ref[j] = v;
+}
+
+// Zero-padded echo samples of lines [base, base + kChunk).
+__global__ void k_rc_load(const float2* __restrict__ raw, int nlines, int stride, int nsamps,
+                          int base, float2* __restrict__ buf) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;   // sample
+  const int i = blockIdx.y;                              // line in chunk
+  const int l = base + i;
+  float2 v = make_float2(0.f, 0.f);
+  if (l < nlines && j < nsamps) v = raw[(size_t)l * stride + kHdr + j];
+  buf[(size_t)i * kFftLen + j] = v;
+}
+
+// Spectrum times conj(reference spectrum).
+__global__ void k_rc_mul(float2* __restrict__ buf, const float2* __restrict__ ref) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = blockIdx.y;
+  float2* p = buf + (size_t)i * kFftLen + j;
+  const float2 a = *p, r = ref[j];
+  *p = make_float2(a.x * r.x + a.y * r.y, a.y * r.x - a.x * r.y);
+}
+
+// Output stage, one block per line.
+constexpr int kStoreBlock = 256;
+__global__ void __launch_bounds__(kStoreBlock)
+k_rc_store(const float2* __restrict__ raw, int nlines, int stride, int base,
+           const float2* __restrict__ buf, int nvalid, float2* __restrict__ out,
+           int out_stride, uint32_t* __restrict__ packed, float* __restrict__ scale,
+           int dtype) {
+  const int i = blockIdx.x;
+  const int l = base + i;
+  if (l >= nlines) return;
+  const float2* row = buf + (size_t)i * kFftLen;
+  if (out) {
+    float2* o = out + (size_t)l * out_stride;
+    for (int k = threadIdx.x; k < kHdr + nvalid; k += kStoreBlock)
+      o[k] = k < kHdr ? raw[(size_t)l * stride + k] : row[k - kHdr];
+  }
+  if (packed) {
+    __shared__ float part[kStoreBlock / 64];
+    float m = 0.f;
+    for (int k = threadIdx.x; k < nvalid; k += kStoreBlock) {
+      const float2 v = row[k];
+      m = fmaxf(m, fmaxf(fabsf(v.x), fabsf(v.y)));
+    }
+    for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o, 64));
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = m;
+    __syncthreads();
+    float mx = part[0];
+    for (int w = 1; w < kStoreBlock / 64; ++w) mx = fmaxf(mx, part[w]);
+    const float sc = pulse_scale(mx);
+    const float inv = 1.0f / sc;
+    if (threadIdx.x == 0) scale[l] = sc;
+    uint32_t* po = packed + (size_t)l * (kHdr + nvalid);
+    for (int k = threadIdx.x; k < kHdr + nvalid; k += kStoreBlock) {
+      uint32_t w = 0u;
+      if (k >= kHdr) {
+        const float2 v = row[k - kHdr];
+        w = pack_word(v.x * inv, v.y * inv, dtype);
+      }
+      po[k] = w;
+    }
+  }
+}
+
+// ------------------------------------------------------------ state
+
+// Per device: the transform buffer of one batch of lines and the reference
+// spectrum of the last chirp seen.
+struct RcState {
+  float2* buf = nullptr;   // [kChunk][kFftLen]
+  float2* ref = nullptr;   // [kFftLen]
+  tdbp_chirp ref_of;
+  bool ref_valid = false;
+};
+
+std::mutex g_rc_mu;
+RcState g_rc[kMaxDev];
+
+int state_for(int dev, RcState** out) {
+  if (dev < 0 || dev >= kMaxDev) return set_error(TDBP_ERR_ARG, "device %d", dev);
+  RcState& r = g_rc[dev];
+  *out = &r;
+  if (r.buf) return TDBP_OK;
+  if (hipMalloc((void**)&r.ref, sizeof(float2) * (size_t)kFftLen) != hipSuccess)
+    return set_error(TDBP_ERR_NOMEM, "range compression buffers");
+  if (hipMalloc((void**)&r.buf, sizeof(float2) * (size_t)kChunk * kFftLen) != hipSuccess) {
+    r.buf = nullptr;
+    return set_error(TDBP_ERR_NOMEM, "range compression buffers");
+  }
+  return TDBP_OK;
+}
+
+bool same_chirp(const tdbp_chirp& a, const tdbp_chirp& b) {
+  return std::memcmp(&a, &b, sizeof a) == 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int tdbp_chirp_decode(const uint8_t* h, tdbp_chirp* c) {
+  if (!h || !c) return set_error(TDBP_ERR_ARG, "chirp decode: null pointer");
+  std::memset(c, 0, sizeof *c);
+  c->rangedecimation = h[40];
+  const double fs = sample_frequency(c->rangedecimation);
+  if (fs == 0.0)
+    return set_error(TDBP_ERR_ARG, "range decimation code %d has no sampling rate",
+                     c->rangedecimation);
+  c->samplefreq = (float)fs;
+  const int w42 = be16(h + 42), w44 = be16(h + 44);
+  const int sgn42 = (w42 & 32768) ? 1 : -1;   // (-1)**(1-ipolarity)
+  const int sgn44 = (w44 & 32768) ? 1 : -1;
+  c->ramprate = (float)((double)(sgn42 * (w42 & 32767)) * kFref * kFref / 2097152.0);
+  c->startfreq = (float)((double)(c->ramprate / 4.f) / kFref +
+                         (double)(sgn44 * (w44 & 32767)) * kFref / 16384.0);
+  c->pulselength = (float)((double)be24(h + 46) / kFref);
+  c->nsamps = be16(h + 65) * 2;
+  const float np = c->pulselength * c->samplefreq;
+  c->npts = (int)np;
+  c->nvalid = c->nsamps - c->npts;
+  return TDBP_OK;
+}
+
+int tdbp_range_compress(const void* d_raw, int nlines, int raw_stride, const tdbp_chirp* c,
+                        void* d_out, int out_stride, void* d_packed, float* d_scale,
+                        int dtype, void* stream) {
+  if (!d_raw || !c || nlines < 1)
+    return set_error(TDBP_ERR_ARG, "range compress: null input or no lines");
+  if (c->nsamps < 1 || c->nsamps > kFftLen || c->nsamps > raw_stride - kHdr)
+    return set_error(TDBP_ERR_ARG, "range compress: %d echo samples for lines of %d (FFT %d)",
+                     c->nsamps, raw_stride, kFftLen);
+  if (c->npts < 1 || c->npts > kFftLen || c->nvalid < 1)
+    return set_error(TDBP_ERR_ARG, "range compress: chirp of %d samples, %d valid",
+                     c->npts, c->nvalid);
+  if (!d_out && !d_packed)
+    return set_error(TDBP_ERR_ARG, "range compress: no output");
+  if (d_out && out_stride < kHdr + c->nvalid)
+    return set_error(TDBP_ERR_ARG, "range compress: output stride %d < %d", out_stride,
+                     kHdr + c->nvalid);
+  if (d_packed && (!d_scale || (dtype != TDBP_CF16 && dtype != TDBP_CBF16)))
+    return set_error(TDBP_ERR_ARG, "range compress: packed output needs CF16/CBF16 and scales");
+  int st = current_device_ready();
+  if (st) return st;
+  int dev = 0;
+  (void)hipGetDevice(&dev);
+  std::lock_guard<std::mutex> lk(g_rc_mu);
+  RcState* r = nullptr;
+  if ((st = state_for(dev, &r))) return st;
+  hipStream_t s = (hipStream_t)stream;
+  if (!r->ref_valid || !same_chirp(r->ref_of, *c)) {
+    hipLaunchKernelGGL(k_chirp, dim3(kFftLen / 256), dim3(256), 0, s, r->ref, c->npts,
+                       c->samplefreq, c->startfreq, c->ramprate);
+    if ((st = hip_check(hipGetLastError(), "chirp launch")) ||
+        (st = fft_run(kFftLen, 1, false, r->ref, s)))
+      return st;
+    r->ref_of = *c;
+    r->ref_valid = true;
+  }
+  const float2* raw = (const float2*)d_raw;
+  for (int base = 0; base < nlines; base += kChunk) {
+    hipLaunchKernelGGL(k_rc_load, dim3(kFftLen / 256, kChunk), dim3(256), 0, s, raw, nlines,
+                       raw_stride, c->nsamps, base, r->buf);
+    if ((st = hip_check(hipGetLastError(), "load launch")) ||
+        (st = fft_run(kFftLen, kChunk, false, r->buf, s)))
+      return st;
+    hipLaunchKernelGGL(k_rc_mul, dim3(kFftLen / 256, kChunk), dim3(256), 0, s, r->buf,
+                       r->ref);
+    if ((st = hip_check(hipGetLastError(), "product launch")) ||
+        (st = fft_run(kFftLen, kChunk, true, r->buf, s)))
+      return st;
+    const int nchunk = nlines - base < kChunk ? nlines - base : kChunk;
+    hipLaunchKernelGGL(k_rc_store, dim3(nchunk), dim3(kStoreBlock), 0, s, raw, nlines,
+                       raw_stride, base, r->buf, c->nvalid, (float2*)d_out, out_stride,
+                       (uint32_t*)d_packed, d_scale, dtype);
+    if ((st = hip_check(hipGetLastError(), "store launch"))) return st;
+  }
+  return TDBP_OK;
+}
+
+}  // extern "C"

@@ -66,6 +66,17 @@ class Geom(C.Structure):
     ]
 
 
+class Chirp(C.Structure):
+    """Mirror of tdbp_chirp."""
+
+    _fields_ = [("rangedecimation", C.c_int32), ("nsamps", C.c_int32), ("npts", C.c_int32),
+                ("nvalid", C.c_int32), ("samplefreq", C.c_float), ("ramprate", C.c_float),
+                ("startfreq", C.c_float), ("pulselength", C.c_float)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 class Samples(C.Structure):
     """Mirror of tdbp_samples."""
 
@@ -111,6 +122,14 @@ def _sig(lib):
         "tdbp_setazoff": (C.c_int, [_GP, _VP, _VP, _VP]),
         "tdbp_pair_count": (C.c_int, [_GP, _VP, _VP, _VP]),
         "tdbp_pack_samples": (C.c_int, [_VP, C.c_int, C.c_int, C.c_int, _VP, _VP, _VP]),
+        "tdbp_chirp_decode": (C.c_int, [_VP, C.POINTER(Chirp)]),
+        "tdbp_crossmul": (C.c_int, [_VP, _VP, C.c_int, C.c_int, C.c_float, C.c_int, C.c_int,
+                                    C.c_int, _VP, _VP, _VP]),
+        "tdbp_makecc": (C.c_int, [_VP, _VP, C.c_int, C.c_int, _VP, _VP]),
+        "tdbp_synth_raw": (C.c_int, [_VP, C.c_int, C.c_int, C.POINTER(Chirp), _VP, _DP, _DP,
+                                     _VP, C.c_int, C.c_float, C.c_uint64, _VP]),
+        "tdbp_range_compress": (C.c_int, [_VP, C.c_int, C.c_int, C.POINTER(Chirp), _VP,
+                                          C.c_int, _VP, _VP, C.c_int, _VP]),
         "tdbp_pixelint": (C.c_int, [_GP, _VP, _VP, _VP, _VP, _VP, _VP]),
         "tdbp_backproject": (C.c_int, [_GP, _SP, _VP, _VP, _VP, _VP, C.c_int, _VP]),
         "tdbp_backproject_variant": (C.c_int, [_GP, _SP, _VP, _VP, _VP, _VP, C.c_int,
@@ -156,7 +175,8 @@ def exported_symbols():
         "tdbp_stream_sync", "tdbp_device_sync", "tdbp_event_create", "tdbp_event_destroy",
         "tdbp_event_record", "tdbp_event_elapsed_ms", "tdbp_mem_info", "tdbp_window",
         "tdbp_orbit_interp", "tdbp_setxyz", "tdbp_fit_aperture", "tdbp_setazoff",
-        "tdbp_pair_count", "tdbp_pack_samples", "tdbp_pixelint", "tdbp_backproject",
+        "tdbp_pair_count", "tdbp_pack_samples", "tdbp_chirp_decode", "tdbp_range_compress", "tdbp_synth_raw", "tdbp_crossmul",
+        "tdbp_makecc", "tdbp_pixelint", "tdbp_backproject",
         "tdbp_backproject_variant", "tdbp_autotune", "tdbp_set_variant",
         "tdbp_variant_count", "tdbp_variant_info", "tdbp_batch_create", "tdbp_batch_add",
         "tdbp_batch_run", "tdbp_batch_set_merge", "tdbp_merge_geo", "tdbp_compare", "tdbp_dropin_timing", "tdbp_batch_pairs",
@@ -441,6 +461,60 @@ def pack_into(d_cf32: DeviceArray, L: int, S: int, dtype: int, d_packed: DeviceA
                                   _s(stream)))
 
 
+def chirp_decode(header: bytes) -> Chirp:
+    """Chirp parameters from an 80-byte packet header (host code, no device)."""
+    h = np.frombuffer(bytes(header), np.uint8)
+    if h.size < 80:
+        raise ValueError("header needs 80 bytes")
+    c = Chirp()
+    check(lib().tdbp_chirp_decode(h.ctypes.data, C.byref(c)))
+    return c
+
+
+def range_compress(d_raw: DeviceArray, chirp: Chirp, d_out: Optional[DeviceArray] = None,
+                   packed=None, stream=None):
+    """Range compression of d_raw (cf32 [nlines][raw_stride], header in the
+    first 10 samples).  d_out: cf32 [nlines][>= nvalid + 10]; packed: (dtype,
+    d_words [nlines][nvalid + 10], d_scale [nlines])."""
+    nlines, stride = d_raw.shape
+    if d_out is not None and (d_out.shape[0] != nlines or d_out.dtype != np.complex64):
+        raise ValueError("output must be cf32 with one row per line")
+    dt, pw, ps = packed if packed is not None else (CF16, None, None)
+    if pw is not None and (pw.nbytes < nlines * (chirp.nvalid + 10) * 4 or
+                           ps.nbytes < nlines * 4):
+        raise ValueError("packed output buffers too small")
+    check(lib().tdbp_range_compress(
+        d_raw.ptr, nlines, stride, C.byref(chirp), d_out.ptr if d_out is not None else None,
+        d_out.shape[1] if d_out is not None else 0, pw.ptr if pw is not None else None,
+        ps.ptr if ps is not None else None, dt, _s(stream)))
+
+
+def crossmul(d_img1: DeviceArray, d_img2: DeviceArray, nvalid: Optional[int] = None,
+             scale: float = 1.0, looksac: int = 1, looksdn: int = 1, mask: bool = True,
+             stream=None):
+    """Interferogram and amplitude of two cf32 images [lines][na]
+    (tdbp_crossmul); returns (d_igram, d_amp) [nvalid/looksdn][na/looksac]."""
+    nl, na = d_img1.shape
+    if d_img2.shape != d_img1.shape:
+        raise ValueError("images differ in shape")
+    nvalid = nl if nvalid is None else nvalid
+    if nvalid > nl:
+        raise ValueError(f"{nvalid} lines of a {nl}-line image")
+    ig = DeviceArray((nvalid // looksdn, na // looksac), np.complex64)
+    am = DeviceArray((nvalid // looksdn, na // looksac), np.complex64)
+    check(lib().tdbp_crossmul(d_img1.ptr, d_img2.ptr, na, nvalid, scale, looksac, looksdn,
+                              1 if mask else 0, ig.ptr, am.ptr, _s(stream)))
+    return ig, am
+
+
+def makecc(d_igram: DeviceArray, d_amp: DeviceArray, stream=None) -> DeviceArray:
+    """Amplitude and correlation rows [nlines][2 npix] (tdbp_makecc)."""
+    nl, npix = d_igram.shape
+    out = DeviceArray((nl, 2 * npix), np.float32)
+    check(lib().tdbp_makecc(d_igram.ptr, d_amp.ptr, npix, nl, out.ptr, _s(stream)))
+    return out
+
+
 def backproject(g: BurstGeometry, smp: Samples, d_satloc, d_dem, d_out,
                 precision=PLAIN_FP64, d_coef=None, variant=None, stream=None):
     gc = g.to_c()
@@ -540,6 +614,20 @@ def compare(d_img: DeviceArray, d_ref: DeviceArray, stream=None) -> dict:
     return {"phase_rms": math.sqrt(sums[0] / w) if w > 0 else 0.0,
             "amp_rms": math.sqrt(sums[2] / w) if w > 0 else 0.0,
             "pixels": int(sums[3]), "extra_nonzero": int(sums[4])}
+
+
+def synth_raw(d_raw: DeviceArray, chirp: Chirp, header: bytes, delay, carrier, amp,
+              noise: float, seed: int, stream=None):
+    """Decoded echo lines of point targets (tdbp_synth_raw)."""
+    L, stride = d_raw.shape
+    dl = np.ascontiguousarray(delay, np.float64).reshape(L, -1)
+    cr = np.ascontiguousarray(carrier, np.float64).reshape(L, -1)
+    a = np.asarray(amp, np.complex128).ravel()
+    af = np.ascontiguousarray(np.stack([a.real, a.imag], 1), np.float32)
+    h = np.frombuffer(bytes(header), np.uint8)
+    dp = lambda x: x.ctypes.data_as(_DP)  # noqa: E731
+    check(lib().tdbp_synth_raw(d_raw.ptr, L, stride, C.byref(chirp), h.ctypes.data, dp(dl),
+                               dp(cr), af.ctypes.data, len(a), noise, seed, _s(stream)))
 
 
 def synth_clutter(d: DeviceArray, L: int, S: int, seed: int, sigma: float,

@@ -125,3 +125,14 @@ end program call_tdbp
         assert r.returncode == 0, r.stderr
         r = subprocess.run([exe], capture_output=True, text=True, timeout=60)
         assert r.returncode == 0 and r.stdout.strip() == "0"
+
+
+def test_howmanygpus_prints_one_integer():
+    """sentinel_gpu.py:98-100 reads the first word of its stdout as the GPU
+    count; without a GPU the query fails and the tool prints 0."""
+    exe = os.path.join(ROOT, "srg_amd", "bin", "howmanygpus")
+    assert os.access(exe, os.X_OK), "build first (make -C srg_amd/csrc)"
+    out = subprocess.run([exe], capture_output=True, text=True, timeout=60)
+    assert out.returncode == 0
+    words = out.stdout.split()
+    assert len(words) == 1 and int(words[0]) >= 0

@@ -1,0 +1,97 @@
+"""GPU tests of interferogram formation (int/crossmul.f90, int/makecc.f90 on
+device images): against the FP64 oracle with looks and masks, an image with
+itself, makecc, and the stack's merged .geo grids of two acquisitions turned
+into an interferogram whose point targets carry zero phase (each image is
+focused with its own orbit, so the geometric phase cancels)."""
+import numpy as np
+import pytest
+
+from oracle import igram_oracle as IO
+from srg_amd import stack as ST
+from srg_amd import synth
+from srg_amd import tdbp as T
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    return float(np.sqrt(np.sum(np.abs(a - b) ** 2) / np.sum(np.abs(b) ** 2)))
+
+
+@pytest.fixture(scope="module")
+def images():
+    rng = np.random.default_rng(9)
+    a = (rng.standard_normal((120, 4000)) + 1j * rng.standard_normal((120, 4000))).astype(
+        np.complex64)
+    b = (0.6 * a + 0.8 * (rng.standard_normal(a.shape) + 1j * rng.standard_normal(a.shape))
+         ).astype(np.complex64)
+    a[:40, :100] = 0
+    b[80:, 3900:] = 0
+    return a, b
+
+
+@pytest.mark.parametrize("looks", [(1, 1), (4, 2), (5, 8)])
+def test_crossmul_matches_oracle(gpu, images, looks):
+    a, b = images
+    la, ld = looks
+    ig, am = T.crossmul(T.DeviceArray.from_host(a), T.DeviceArray.from_host(b), nvalid=117,
+                        scale=1e-3, looksac=la, looksdn=ld)
+    T.sync()
+    gi, ga = ig.download(), am.download()
+    oi, oa = IO.crossmul(a, b, nvalid=117, scale=1e-3, looksac=la, looksdn=ld)
+    assert gi.shape == oi.shape == (117 // ld, 4000 // la)
+    np.testing.assert_array_equal(gi == 0, oi == 0)            # the same masks
+    assert rel(gi, oi) < 1e-5 and rel(ga, oa) < 1e-5, (rel(gi, oi), rel(ga, oa))
+
+
+def test_image_with_itself_and_makecc(gpu, images):
+    a, _ = images
+    d = T.DeviceArray.from_host(a)
+    ig, am = T.crossmul(d, d, looksac=4, looksdn=4)
+    cc = T.makecc(ig, am)
+    T.sync()
+    gi, ga, gc = ig.download(), am.download(), cc.download()
+    m = gi != 0
+    assert np.all(np.abs(gi.imag[m]) <= 1e-5 * gi.real[m])
+    np.testing.assert_array_equal(ga.real, ga.imag)
+    w = gi.shape[1]
+    np.testing.assert_allclose(gc[:, w:][m], 1.0, rtol=1e-5)
+    np.testing.assert_allclose(gc, IO.makecc(gi.astype(np.complex128), ga.astype(np.complex128)),
+                               rtol=1e-6, atol=1e-30)
+
+
+def test_stack_interferogram_targets_zero_phase(gpu):
+    stk = synth.ts_stack(nacq=2, nburst=3, lines=4500, W=1500)
+    d_dem = T.DeviceArray.from_host(stk.grid.dem)
+    bursts = [ST.StackBurst(a, sc.geom, sc.satloc,
+                            d_dem.rows(sc.geom.firstline, sc.geom.nlines), payload=sc)
+              for a, k, sc in stk.items()]
+    grids = [T.DeviceArray((stk.grid.lines, stk.grid.W), np.complex64) for _ in range(2)]
+    for g in grids:
+        g.zero()
+
+    def load(i, b, r):
+        g = b.geom
+        tmp = T.DeviceArray((g.L, g.S), np.complex64)
+        synth.device_samples(b.payload, tmp, r.sat)
+        T.pack_into(tmp, g.L, g.S, T.CF16, r.buffers[0], r.buffers[1])
+        T.sync()
+        tmp.free()
+
+    ST.run_stack(bursts, grids, load, precision=T.MIXED, dtype=T.CF16)
+    ig, am = T.crossmul(grids[0], grids[1])
+    T.sync()
+    gi = ig.download()
+    oi, _ = IO.crossmul(grids[0].download(), grids[1].download())
+    assert rel(gi, oi) < 1e-5
+    checked = 0
+    for k, sc in enumerate(stk.bursts[0]):
+        for (r, c) in sc.targets_px:
+            row = sc.geom.firstline + r
+            v = gi[row, c]
+            nb = np.abs(gi[row, max(c - 30, 0):c + 31])
+            if abs(v) < 20 * np.median(nb):
+                continue          # row taken over by a neighbouring burst's window
+            assert abs(np.angle(v)) < 0.05, (k, r, c, v)
+            checked += 1
+    assert checked >= 6

@@ -284,3 +284,12 @@ def test_pack_roundtrip(gpu):
         err = np.maximum(np.abs(y.real - x.real), np.abs(y.imag - x.imag))
         assert np.all(err <= eps * pmax + 1e-30)
         assert sc[5] == 1.0 and np.all(y[5] == 0)
+
+
+def test_howmanygpus_counts_the_gpu(gpu):
+    import subprocess
+
+    exe = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "srg_amd",
+                       "bin", "howmanygpus")
+    out = subprocess.run([exe], capture_output=True, text=True, timeout=60)
+    assert out.returncode == 0 and int(out.stdout.split()[0]) >= 1

@@ -1,0 +1,131 @@
+"""GPU tests of range compression on the device: against the FP64 oracle
+(300 lines: one full and one partial 256-line transform batch), the fused
+FP16/BF16 store equal to packing the cf32 output bit for bit, the device echo
+synthesis against its numpy twin, argument errors, and raw echoes of point
+targets through range compression and the mixed backprojection focusing at
+their pixels with their phase."""
+import math
+
+import numpy as np
+import pytest
+
+from oracle import range_oracle as RO
+from srg_amd import raw, synth
+from srg_amd import tdbp as T
+
+pytestmark = pytest.mark.gpu
+
+SMALL = raw.HeaderCodes(nsamps=6000)        # IW1 chirp, a short echo window
+
+
+@pytest.fixture(scope="module")
+def lines300():
+    rng = np.random.default_rng(11)
+    h = raw.header_bytes(raw.IW1)
+    c = T.chirp_decode(h)
+    x = ((rng.standard_normal((300, raw.RAW_STRIDE)) +
+          1j * rng.standard_normal((300, raw.RAW_STRIDE))) * 0.5).astype(np.complex64)
+    x[:, c.nsamps + raw.HDR:] = 0
+    x[:, :raw.HDR] = raw.header_samples(h)
+    return h, c, x
+
+
+def test_range_compress_matches_oracle(gpu, lines300):
+    h, c, x = lines300
+    d_raw = T.DeviceArray.from_host(x)
+    d_out = T.DeviceArray((x.shape[0], c.nvalid + 10), np.complex64)
+    T.range_compress(d_raw, c, d_out)
+    T.sync()
+    got = d_out.download()
+    want = RO.compress(x, RO.chirp_params(h))
+    assert np.array_equal(got[:, :10].view(np.uint32), x[:, :10].view(np.uint32))
+    d = got[:, 10:].astype(np.complex128) - want[:, 10:]
+    rel = math.sqrt(np.sum(np.abs(d) ** 2) / np.sum(np.abs(want[:, 10:]) ** 2))
+    print(f"range compression vs FP64 oracle: relative RMS error {rel:.2e}")
+    assert rel < 1e-5
+    # repeat: the cached reference spectrum gives the same bits
+    T.range_compress(d_raw, c, d_out)
+    T.sync()
+    assert np.array_equal(d_out.download().view(np.uint32), got.view(np.uint32))
+
+
+@pytest.mark.parametrize("dtype", [T.CF16, T.CBF16])
+def test_fused_pack_equals_pack_of_cf32(gpu, lines300, dtype):
+    h, c, x = lines300
+    n = x.shape[0]
+    S = c.nvalid + 10
+    d_raw = T.DeviceArray.from_host(x)
+    d_out = T.DeviceArray((n, S), np.complex64)
+    pw = T.DeviceArray((n, S), np.uint32)
+    ps = T.DeviceArray((n,), np.float32)
+    T.range_compress(d_raw, c, d_out, packed=(dtype, pw, ps))
+    T.sync()
+    cf = d_out.download()
+    cf[:, :10] = 0                                   # backprojectgpusub.f90:190
+    ref_w, ref_s = T.pack(T.DeviceArray.from_host(cf), n, S, dtype)
+    T.sync()
+    assert np.array_equal(pw.download(), ref_w.download())
+    assert np.array_equal(ps.download().view(np.uint32), ref_s.download().view(np.uint32))
+    # packed-only call gives the same words
+    pw2 = T.DeviceArray((n, S), np.uint32)
+    ps2 = T.DeviceArray((n,), np.float32)
+    T.range_compress(d_raw, c, None, packed=(dtype, pw2, ps2))
+    T.sync()
+    assert np.array_equal(pw2.download(), pw.download())
+
+
+def test_range_compress_argument_errors(gpu, lines300):
+    h, c, x = lines300
+    d_raw = T.DeviceArray.from_host(x[:4, :c.nsamps])          # stride too short
+    with pytest.raises(T.TdbpError):
+        T.range_compress(d_raw, c, T.DeviceArray((4, c.nvalid + 10), np.complex64))
+    d_raw = T.DeviceArray.from_host(x[:4])
+    with pytest.raises(T.TdbpError):
+        T.range_compress(d_raw, c, None)                       # no output
+    with pytest.raises(T.TdbpError):
+        T.range_compress(d_raw, c, T.DeviceArray((4, c.nvalid), np.complex64))
+
+
+def echo_scene():
+    h = raw.header_bytes(SMALL)
+    c = T.chirp_decode(h)
+    S = c.nvalid + 10
+    sc = synth.make_burst(600, S, 128, 128, dlat=-0.0002, dlon=0.0002, aperture=256,
+                          dem="flat", ntargets=4, clutter_sigma=0.0, seed=21)
+    return h, c, sc
+
+
+def test_device_echoes_match_numpy(gpu):
+    h, c, sc = echo_scene()
+    host = raw.synth_lines(sc, c, h)
+    d = T.DeviceArray((sc.L, raw.RAW_STRIDE), np.complex64)
+    raw.device_lines(sc, c, h, d)
+    got = d.download()
+    assert np.array_equal(got[:, :10].view(np.uint32), host[:, :10].view(np.uint32))
+    err = np.abs(got[:, 10:] - host[:, 10:]).max()
+    assert err < 1e-4 * max(1.0, np.abs(host).max())
+    assert np.all(got[:, 10 + c.nsamps:] == 0)
+
+
+def test_raw_echoes_focus_after_range_compression(gpu):
+    h, c, sc = echo_scene()
+    g = sc.geom
+    d_raw = T.DeviceArray((sc.L, raw.RAW_STRIDE), np.complex64)
+    raw.device_lines(sc, c, h, d_raw, noise=0.05, seed=4)
+    pw = T.DeviceArray((sc.L, g.S), np.uint32)
+    ps = T.DeviceArray((sc.L,), np.float32)
+    T.range_compress(d_raw, c, None, packed=(T.CF16, pw, ps))
+    d_sat = T.DeviceArray.from_host(sc.satloc)
+    d_dem = T.DeviceArray.from_host(sc.dem)
+    out = T.DeviceArray((g.nlines, g.demwidth), np.complex64)
+    T.backproject(g, T.samples(pw, T.CF16, ps), d_sat, d_dem, out, T.MIXED)
+    T.sync()
+    img = out.download()
+    assert len(sc.targets_px) == 4
+    for (r, col), a in zip(sc.targets_px, sc.targets_amp):
+        v = img[r, col]
+        # compressed peak height N = 32768 per unit echo, averaged over pulses
+        assert abs(v) > 0.5 * RO.NFFT, (r, col, v)
+        assert abs(np.angle(v * np.conj(a))) < 0.05, (r, col, v, a)
+        nb = img[r, max(col - 20, 0):col + 21]
+        assert abs(v) == np.abs(nb).max()
