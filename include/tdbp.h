@@ -274,6 +274,79 @@ int tdbp_merge_geo(void *d_grid, int W, int grid_lines, int firstline, int nline
 int tdbp_compare(const void *d_img, const void *d_ref, size_t n, double *sums,
                  void *stream);
 
+/* ------------------------------------------- burst set-up (the caller side) */
+
+/* Burst parameters from the first line's 80-byte header
+ * (sentinel/backprojectgpusub.f90:406-439, burstparams): start time (bytes
+ * 68-75, native double), PRF, sampling rate (Hz), PRI (s), range window
+ * start time SWST (s), rank, range0 = c/2 (rank PRI + SWST) + 60 m, and the
+ * wavelength literal 0.05546576 as the caller stores it (FP32-rounded). */
+typedef struct tdbp_burst_params {
+  double starttime, prf, samplefreq, pri, swst, range0, wvl;
+  int32_t rangedecimation, rank;
+} tdbp_burst_params;
+
+int tdbp_burst_decode(const uint8_t *header80, tdbp_burst_params *p);
+
+/* TOPS steering coefficients (fdopcoefs + linfit,
+ * sentinel/backprojectgpusub.f90:262-300, :355-383): least-squares line of
+ * the azimuth beam angle against time over lines 21..lines-20.  headers:
+ * `lines` headers of 80 bytes, stride_bytes apart (a decoded line is
+ * 30000 x 8 bytes).  Outputs the FP32 coefficients the reference keeps. */
+int tdbp_steering_fit(const uint8_t *headers, size_t stride_bytes, int lines,
+                      double *angc0, double *angc1);
+
+/* Integration length of sentinel/backprojectgpusub.f90:193-209 from the
+ * satellite velocity at the burst's middle line (satvel(:, lines/2)). */
+int tdbp_aperture(const double *vel_mid_burst, double range0, double wvl, double prf,
+                  double angc1);
+
+/* Ground footprint of a burst (sentinel/bounds.f90): min/max latitude and
+ * longitude (degrees) of the four corners at heights -100 m and 2000 m for
+ * times [startime, endtime] and ranges [startr, endr]; the driver calls it
+ * with tstart - 2 s, tend + 2 s (:232). */
+int tdbp_bounds(double startime, double endtime, double startr, double endr,
+                const double *timeorbit, const double *xx, const double *vv, int nsv,
+                double *latlons);
+
+/* The driver's per-burst set-up (sentinel/backprojectgpusub.f90:140-247)
+ * from the burst's line headers (`lines` headers, header_stride bytes apart;
+ * the first line's header gives the burst parameters), the satellite
+ * velocity of line lines/2 (1-based, as the driver takes it from the position
+ * file), the orbit state vectors and the DEM grid: fills g (integration
+ * length, range window rngstart = (2 range0 / c) c / 2, rngend, dmrg, wvl,
+ * prf, steering coefficients, tmid and the orbit at tmid, the DEM window from
+ * the footprint) and latlons; *in_scene = 0 where the driver skips the burst
+ * (|latlons| > 1000, :231-232).  The state-vector arrays are referenced by
+ * g, not copied. */
+int tdbp_burst_setup(const uint8_t *headers, size_t header_stride, int lines,
+                     int rangesamples, const double *satvel_mid, const double *timeorbit,
+                     const double *xx, const double *vv, int nsv, int demwidth,
+                     int demlength, double firstlat, double firstlon, double deltalat,
+                     double deltalon, tdbp_geom *g, double *latlons, int *in_scene);
+
+/* The 80 header bytes of each of nlines decoded lines on the device (cf32
+ * rows of `stride` samples) into host memory [nlines][80].  Synchronises. */
+int tdbp_headers_dtoh(const void *d_lines, int nlines, int stride, uint8_t *host,
+                      void *stream);
+
+/* Host headers [nlines][80] into the first 80 bytes of each device line. */
+int tdbp_headers_htod(void *d_lines, int nlines, int stride, const uint8_t *host,
+                      void *stream);
+
+/* Per-pulse satellite positions and velocities of decoded lines on the
+ * device (sentinel/sentineltimingsub.f90:55-66): each line's time from its
+ * header bytes 68-75, 4-point Hermite on the state vectors from
+ * int((t - t1)/dt) - 1 (1-based).  d_lines: cf32 [nlines][stride] on the
+ * device; outputs d_satloc, d_satvel f64 [nlines][3], d_times f64 [nlines].
+ * quantize != 0 rounds positions to 1e-7 m and velocities to 1e-8 m/s, the
+ * values the driver reads back from the f16.7 / f14.8 position file.  A
+ * line time outside the state vectors' interpolation range is an error.
+ * Synchronises the stream. */
+int tdbp_pulse_orbit(const void *d_lines, int nlines, int stride, const double *timeorbit,
+                     const double *xx, const double *vv, int nsv, int quantize,
+                     double *d_satloc, double *d_satvel, double *d_times, void *stream);
+
 /* ------------------------------------------------- range compression */
 
 /* Chirp and echo-window parameters of a swath, decoded from the 80-byte
@@ -300,7 +373,9 @@ int tdbp_chirp_decode(const uint8_t *header80, tdbp_chirp *c);
  * Outputs (either or both):
  *   d_out     cf32 [nlines][out_stride >= nvalid + 10]: the header samples,
  *             then nvalid compressed samples (the reference rangeprocdata
- *             layout, :121-122);
+ *             layout, :121-122); with flags & TDBP_RC_ZERO_HEADER the header
+ *             samples are written as zero, as the burst driver leaves them
+ *             before backprojection (sentinel/backprojectgpusub.f90:190);
  *   d_packed  CF16/CBF16 words [nlines][nvalid + 10] and d_scale [nlines]:
  *             the backprojection input (header samples zero, as
  *             sentinel/backprojectgpusub.f90:190 leaves them) in the storage
@@ -308,8 +383,9 @@ int tdbp_chirp_decode(const uint8_t *header80, tdbp_chirp *c);
  * The transform is the reference's: 32768-point forward FFT of the
  * zero-padded echo, times conj(FFT(reference chirp / npts)), inverse FFT
  * without normalisation, in FP32. */
+enum { TDBP_RC_ZERO_HEADER = 1 };
 int tdbp_range_compress(const void *d_raw, int nlines, int raw_stride,
-                        const tdbp_chirp *c, void *d_out, int out_stride,
+                        const tdbp_chirp *c, void *d_out, int out_stride, int flags,
                         void *d_packed, float *d_scale, int dtype, void *stream);
 
 /* ------------------------------------------ interferograms (.geo stacks) */

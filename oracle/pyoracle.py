@@ -21,6 +21,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ORACLE_SO = os.path.join(HERE, "libtdbp_oracle.so")
 REF_SO = os.path.join(HERE, "_ref", "libref_tdbp_cpu.so")
 REF_O0_SO = os.path.join(HERE, "_ref", "libref_tdbp_cpu_O0.so")
+BOUNDS_SO = os.path.join(HERE, "_ref", "libref_bounds.so")
 
 CPU_SEMANTICS, GPU_SEMANTICS = 0, 1
 
@@ -280,6 +281,30 @@ class RefCPU:
                 db(prf)]
         self.lib.azimuth_compress_cpu_(*args)
         return keep
+
+
+class RefBounds:
+    """bounds_ and latlon_ of the reference compiled from its own Fortran
+    sources (oracle/_ref/libref_bounds.so)."""
+
+    def __init__(self, path: str = BOUNDS_SO):
+        if not os.path.exists(path):
+            raise FileNotFoundError(f"{path} not built (make -C oracle ref)")
+        self.lib = C.CDLL(path)
+
+    def bounds(self, startime, endtime, startr, endr, timeorbit, xx, vv):
+        t = _c(timeorbit, np.float64)
+        x = _c(xx, np.float64)
+        v = _c(vv, np.float64)
+        out = np.zeros(4)
+        d = lambda a: C.byref(C.c_double(float(a)))  # noqa: E731
+        self.lib.bounds_(d(startime), d(endtime), d(startr), d(endr), _p(t, _D), _p(x, _D),
+                         _p(v, _D), C.byref(C.c_int(len(t))), _p(out, _D))
+        return out
+
+
+def bounds_available(path: str = None) -> bool:
+    return os.path.exists(path or BOUNDS_SO)
 
 
 def ref_available(path: str = REF_SO) -> bool:

@@ -355,6 +355,33 @@ int tdbp_compare(const void* d_img, const void* d_ref, size_t n, double* sums,
   return TDBP_OK;
 }
 
+int tdbp_headers_dtoh(const void* d_lines, int nlines, int stride, uint8_t* host, void* stream) {
+  if (!d_lines || !host || nlines < 1 || stride < 10)
+    return set_error(TDBP_ERR_ARG, "headers: bad arguments");
+  int st = current_device_ready();
+  if (st) return st;
+  hipStream_t s = (hipStream_t)stream;
+  if ((st = hip_check(hipMemcpy2DAsync(host, 80, d_lines, (size_t)stride * 8, 80, (size_t)nlines,
+                                       hipMemcpyDeviceToHost, s),
+                      "header download")))
+    return st;
+  return hip_check(hipStreamSynchronize(s), "header sync");
+}
+
+int tdbp_headers_htod(void* d_lines, int nlines, int stride, const uint8_t* host,
+                      void* stream) {
+  if (!d_lines || !host || nlines < 1 || stride < 10)
+    return set_error(TDBP_ERR_ARG, "headers: bad arguments");
+  int st = current_device_ready();
+  if (st) return st;
+  hipStream_t s = (hipStream_t)stream;
+  if ((st = hip_check(hipMemcpy2DAsync(d_lines, (size_t)stride * 8, host, 80, 80, (size_t)nlines,
+                                       hipMemcpyHostToDevice, s),
+                      "header upload")))
+    return st;
+  return hip_check(hipStreamSynchronize(s), "header sync");
+}
+
 int tdbp_dev_copy(void* d, const void* src, size_t n, void* s) {
   return hip_check(launch_copy(d, src, n, (hipStream_t)s), "copy");
 }

@@ -136,7 +136,8 @@ __global__ void k_makecc(const float2* __restrict__ ig, const float2* __restrict
   out[(size_t)l * 2 * npix + npix + i] = cc;
 }
 
-// Per device: the transform buffers of one batch.
+// Per device: the transform buffers of one batch, sized for the largest
+// transform length seen; a call uses its own length nnn (rows nnn apart).
 struct IgState {
   size_t nnn = 0;
   float2* buf = nullptr;   // [kRows][nnn]
@@ -207,18 +208,18 @@ int tdbp_crossmul(const void* d_img1, const void* d_img2, int na, int nvalid, fl
     const int nol = nout - ol0 < lines_per_batch ? nout - ol0 : lines_per_batch;
     const int row0 = ol0 * looksdn;
     const int rows = nol * looksdn;
-    if ((st = upsample(i1, na, rows, row0, used_rows, s.nnn, scale, s, s.up1, str))) return st;
+    if ((st = upsample(i1, na, rows, row0, used_rows, nnn, scale, s, s.up1, str))) return st;
     if (i2 == i1) {
-      if ((st = hip_check(hipMemcpyAsync(s.up2, s.up1, sizeof(float2) * kRows * 2 * s.nnn,
+      if ((st = hip_check(hipMemcpyAsync(s.up2, s.up1, sizeof(float2) * kRows * 2 * nnn,
                                          hipMemcpyDeviceToDevice, str),
                           "upsample copy")))
         return st;
-    } else if ((st = upsample(i2, na, rows, row0, used_rows, s.nnn, scale, s, s.up2, str))) {
+    } else if ((st = upsample(i2, na, rows, row0, used_rows, nnn, scale, s, s.up2, str))) {
       return st;
     }
     const int nw = na / looksac;
     hipLaunchKernelGGL(k_xmul, dim3((unsigned)((nw + 127) / 128), nol), dim3(128), 0, str,
-                       s.up1, s.up2, (int)s.nnn, i1, i2, na, row0, nol, ol0, looksac, looksdn,
+                       s.up1, s.up2, (int)nnn, i1, i2, na, row0, nol, ol0, looksac, looksdn,
                        mask ? 1 : 0, (float2*)d_igram, (float2*)d_amp);
     if ((st = hip_check(hipGetLastError(), "crossmul launch"))) return st;
   }

@@ -111,8 +111,8 @@ constexpr int kStoreBlock = 256;
 __global__ void __launch_bounds__(kStoreBlock)
 k_rc_store(const float2* __restrict__ raw, int nlines, int stride, int base,
            const float2* __restrict__ buf, int nvalid, float2* __restrict__ out,
-           int out_stride, uint32_t* __restrict__ packed, float* __restrict__ scale,
-           int dtype) {
+           int out_stride, int zero_header, uint32_t* __restrict__ packed,
+           float* __restrict__ scale, int dtype) {
   const int i = blockIdx.x;
   const int l = base + i;
   if (l >= nlines) return;
@@ -120,7 +120,8 @@ k_rc_store(const float2* __restrict__ raw, int nlines, int stride, int base,
   if (out) {
     float2* o = out + (size_t)l * out_stride;
     for (int k = threadIdx.x; k < kHdr + nvalid; k += kStoreBlock)
-      o[k] = k < kHdr ? raw[(size_t)l * stride + k] : row[k - kHdr];
+      o[k] = k >= kHdr ? row[k - kHdr]
+                       : (zero_header ? make_float2(0.f, 0.f) : raw[(size_t)l * stride + k]);
   }
   if (packed) {
     __shared__ float part[kStoreBlock / 64];
@@ -209,8 +210,8 @@ int tdbp_chirp_decode(const uint8_t* h, tdbp_chirp* c) {
 }
 
 int tdbp_range_compress(const void* d_raw, int nlines, int raw_stride, const tdbp_chirp* c,
-                        void* d_out, int out_stride, void* d_packed, float* d_scale,
-                        int dtype, void* stream) {
+                        void* d_out, int out_stride, int flags, void* d_packed,
+                        float* d_scale, int dtype, void* stream) {
   if (!d_raw || !c || nlines < 1)
     return set_error(TDBP_ERR_ARG, "range compress: null input or no lines");
   if (c->nsamps < 1 || c->nsamps > kFftLen || c->nsamps > raw_stride - kHdr)
@@ -258,7 +259,8 @@ int tdbp_range_compress(const void* d_raw, int nlines, int raw_stride, const tdb
     const int nchunk = nlines - base < kChunk ? nlines - base : kChunk;
     hipLaunchKernelGGL(k_rc_store, dim3(nchunk), dim3(kStoreBlock), 0, s, raw, nlines,
                        raw_stride, base, r->buf, c->nvalid, (float2*)d_out, out_stride,
-                       (uint32_t*)d_packed, d_scale, dtype);
+                       (flags & TDBP_RC_ZERO_HEADER) ? 1 : 0, (uint32_t*)d_packed, d_scale,
+                       dtype);
     if ((st = hip_check(hipGetLastError(), "store launch"))) return st;
   }
   return TDBP_OK;

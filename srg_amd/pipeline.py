@@ -1,0 +1,93 @@
+"""One burst from decoded echo lines to its .geo rows, on the device: the
+work of sentinel/processsub.f90:96-126 (range compression) and of the burst
+loop of sentinel/backprojectgpusub.f90:140-247 (set-up and
+azimuth_compress), with the per-pulse orbit of sentineltimingsub.f90:44-69
+computed from the line times instead of going through a position file.
+
+Only the 80-byte line headers and one satellite velocity come back to the
+host; the samples, positions and the output stay in HBM.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass
+from typing import Optional
+
+import numpy as np
+
+from . import tdbp as T
+
+
+@dataclass
+class DemGrid:
+    """The scene's DEM on the device (int16 [length][width], north-up) and
+    its grid (the elevation.dem.rsc values the driver reads)."""
+
+    d_dem: T.DeviceArray
+    firstlat: float
+    firstlon: float
+    deltalat: float
+    deltalon: float
+
+    @property
+    def length(self):
+        return self.d_dem.shape[0]
+
+    @property
+    def width(self):
+        return self.d_dem.shape[1]
+
+
+@dataclass
+class BurstResult:
+    geom: Optional[T.BurstGeometry]
+    latlons: np.ndarray
+    in_scene: bool
+    chirp: T.Chirp
+    d_satloc: T.DeviceArray
+    d_out: Optional[T.DeviceArray] = None
+
+
+def process_raw_burst(d_lines: T.DeviceArray, timeorbit, xx, vv, dem: DemGrid,
+                      d_grid: Optional[T.DeviceArray] = None, precision: int = T.MIXED,
+                      dtype: int = T.CF16, variant: Optional[int] = None,
+                      quantize_positions: bool = True, stream=None) -> BurstResult:
+    """Range-compress, set up and backproject one burst of decoded lines
+    (cf32 [lines][stride], headers in the first 10 samples), then merge its
+    window into d_grid (cf32 [dem.length][dem.width]) when given.
+
+    precision/dtype select the kernel and the sample storage (the FP64 kernel
+    reads cf32).  quantize_positions reproduces the driver's position-file
+    round trip (f16.7 / f14.8)."""
+    if precision == T.PLAIN_FP64 and dtype != T.CF32:
+        raise ValueError("the FP64 kernel reads cf32 samples")
+    L, stride = d_lines.shape
+    headers = T.headers_dtoh(d_lines, stream)
+    chirp = T.chirp_decode(headers[0])
+    S = chirp.nvalid + 10
+    d_sat, d_vel, _ = T.pulse_orbit(d_lines, timeorbit, xx, vv, quantize=quantize_positions,
+                                    stream=stream)
+    vmid = d_vel.rows(L // 2 - 1, 1).download()[0]        # satvel(:, rawdatalines/2)
+    geom, ll, ins = T.burst_setup(headers, S, vmid, timeorbit, xx, vv, dem.width, dem.length,
+                                  dem.firstlat, dem.firstlon, dem.deltalat, dem.deltalon)
+    res = BurstResult(geom=geom, latlons=ll, in_scene=ins, chirp=chirp, d_satloc=d_sat)
+    if not ins:
+        return res
+    if dtype == T.CF32:
+        data = T.DeviceArray((L, S), np.complex64)
+        T.range_compress(d_lines, chirp, data, zero_header=True, stream=stream)
+        smp = T.samples(data)
+        keep = [data]
+    else:
+        words = T.DeviceArray((L, S), np.uint32)
+        scale = T.DeviceArray((L,), np.float32)
+        T.range_compress(d_lines, chirp, None, packed=(dtype, words, scale), stream=stream)
+        smp = T.samples(words, dtype, scale)
+        keep = [words, scale]
+    d_win = dem.d_dem.rows(geom.firstline, geom.nlines)
+    out = T.DeviceArray((geom.nlines, geom.demwidth), np.complex64)
+    T.backproject(geom, smp, d_sat, d_win, out, precision, variant=variant, stream=stream)
+    if d_grid is not None:
+        T.merge_geo(d_grid, geom.firstline, out, stream=stream)
+    res.d_out = out
+    res._keep = keep
+    return res

@@ -133,3 +133,120 @@ def device_lines(sc, c: tdbp.Chirp, header: bytes, d_raw: "tdbp.DeviceArray",
     """The same model on the GPU into d_raw (cf32 [L][stride])."""
     m = echo_model(sc, c)
     tdbp.synth_raw(d_raw, c, header, m.delay, m.carrier, m.amp, noise, seed, stream)
+
+
+@dataclass
+class RawScene:
+    """A burst of decoded lines made self-consistent with the driver's own
+    derivation: per-line headers (times, chirp, range window codes), the orbit
+    state vectors, the satellite positions the pipeline will compute, a DEM
+    grid around the footprint, and point targets on grid pixels inside the
+    burst's focusable zone."""
+
+    headers: np.ndarray        # uint8 [L][80]
+    chirp: "tdbp.Chirp"
+    timeorbit: np.ndarray
+    xx: np.ndarray
+    vv: np.ndarray
+    times: np.ndarray
+    satloc: np.ndarray         # [L][3], quantised as the position file
+    dem: np.ndarray            # int16 [length][width]
+    firstlat: float
+    firstlon: float
+    deltalat: float
+    deltalon: float
+    geom: "tdbp.BurstGeometry"
+    targets_xyz: np.ndarray
+    targets_amp: np.ndarray
+    targets_px: np.ndarray     # grid (row, col)
+
+    @property
+    def L(self):
+        return self.headers.shape[0]
+
+
+def _quantize(a, s):
+    ai = np.trunc(a)
+    return (ai * s + np.rint((a - ai) * s)) / s
+
+
+def raw_burst_scene(L: int = 600, nsamps: int = 6000, centre_range: float = 850e3,
+                    posting: float = 0.0004, ntargets: int = 4, seed: int = 0,
+                    t0: float = 43200.0, heading_deg: float = 0.0) -> RawScene:
+    from . import synth
+    rng = np.random.default_rng(seed)
+    c = 299792458.0
+    orbit = synth.make_orbit(35.0, -118.0, t0, heading_deg=heading_deg, seed=seed)
+    base = HeaderCodes(nsamps=nsamps)
+    bp = tdbp.burst_decode(header_bytes(base))
+    chirp = tdbp.chirp_decode(header_bytes(base))
+    S = chirp.nvalid + HDR
+    dmrg = c / 2.0 / bp.samplefreq
+    range0 = centre_range - 0.5 * S * dmrg
+    swst = (range0 - 60.0) * 2.0 / c - base.rank * bp.pri
+    code = int(round(swst / float(np.float32(1e-6)) * FREF))
+    if not 0 <= code < (1 << 24):
+        raise ValueError(f"range window code {code}")
+    prf = bp.prf
+    tstart = t0 - 0.5 * (L - 1) / prf
+    times = tstart + np.arange(L) / prf
+    headers = np.zeros((L, 80), np.uint8)
+    for i in range(L):
+        headers[i] = np.frombuffer(header_bytes(HeaderCodes(nsamps=nsamps, swst=code,
+                                                            starttime=float(times[i]))), np.uint8)
+    times = np.array([struct.unpack("<d", bytes(h[68:76]))[0] for h in headers])
+    tmid = 0.5 * (times[0] + times[0] + (L - 1) / prf)
+    timeorbit, xx, vv = orbit.state_vectors(tmid)
+    sat, vel = tdbp.orbit_interp(timeorbit, xx, vv, times)
+    sat, vel = _quantize(sat, 1e7), _quantize(vel, 1e8)
+    bp = tdbp.burst_decode(headers[0])
+    rngstart = (2 * bp.range0 / c) * c / 2.0
+    rngend = rngstart + (S - 1) * dmrg
+    ll = tdbp.bounds(times[0] - 2.0, times[0] + (L - 1) / prf + 2.0, rngstart, rngend,
+                     timeorbit, xx, vv)
+    m = 4 * posting
+    firstlat = ll[1] + m
+    firstlon = ll[2] - m
+    length = int(math.ceil((ll[1] - ll[0] + 2 * m) / posting))
+    width = int(math.ceil((ll[3] - ll[2] + 2 * m) / posting))
+    dem = np.zeros((length, width), np.int16)
+    geom, ll2, ins = tdbp.burst_setup(headers, S, vel[L // 2 - 1], timeorbit, xx, vv, width,
+                                      length, firstlat, firstlon, -posting, posting)
+    # targets on pixels whose zero-Doppler pulse and range sit well inside
+    A = geom.A
+    rows, cols = [], []
+    for _ in range(5000):
+        if len(rows) == ntargets:
+            break
+        r = int(rng.integers(geom.firstline, geom.firstline + geom.nlines))
+        col = int(rng.integers(geom.firstpix, geom.lastpix + 1))
+        x = synth.llh_to_ecef(firstlat - r * posting, firstlon + col * posting, 0.0)
+        tz = synth.zero_doppler_time(orbit, x, tmid)
+        p = (tz - times[0]) * prf
+        R = float(np.linalg.norm(x - orbit.pos(tz)))
+        i = (R - rngstart) / dmrg
+        if A // 2 + 8 <= p <= L - 1 - A // 2 - 8 and 200 <= i <= S - 200:
+            rows.append(r)
+            cols.append(col)
+    rows = np.array(rows)
+    cols = np.array(cols)
+    txyz = synth.llh_to_ecef(firstlat - rows * posting, firstlon + cols * posting,
+                             np.zeros(len(rows))).reshape(-1, 3)
+    amp = np.exp(1j * rng.uniform(-np.pi, np.pi, len(rows)))
+    return RawScene(headers=headers, chirp=chirp, timeorbit=timeorbit, xx=xx, vv=vv,
+                    times=times, satloc=sat, dem=dem, firstlat=firstlat, firstlon=firstlon,
+                    deltalat=-posting, deltalon=posting, geom=geom, targets_xyz=txyz,
+                    targets_amp=amp, targets_px=np.stack([rows, cols], 1))
+
+
+def device_raw_scene(rs: RawScene, d_lines: "tdbp.DeviceArray", noise: float = 0.0,
+                     seed: int = 0):
+    """The scene's decoded lines into d_lines (cf32 [L][stride]): chirp echoes
+    of the targets from the quantised positions, per-line headers."""
+    g = rs.geom
+    R = np.linalg.norm(rs.satloc[:, None, :] - rs.targets_xyz[None, :, :], axis=2)
+    delay = (R - g.rngstart) / g.dmrg - HDR
+    cyc = R * (2.0 / g.wvl)
+    tdbp.synth_raw(d_lines, rs.chirp, bytes(rs.headers[0]), delay, cyc - np.floor(cyc),
+                   rs.targets_amp, noise, seed)
+    tdbp.headers_htod(d_lines, rs.headers)

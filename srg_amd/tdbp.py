@@ -77,6 +77,17 @@ class Chirp(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class BurstParams(C.Structure):
+    """Mirror of tdbp_burst_params."""
+
+    _fields_ = [("starttime", C.c_double), ("prf", C.c_double), ("samplefreq", C.c_double),
+                ("pri", C.c_double), ("swst", C.c_double), ("range0", C.c_double),
+                ("wvl", C.c_double), ("rangedecimation", C.c_int32), ("rank", C.c_int32)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 class Samples(C.Structure):
     """Mirror of tdbp_samples."""
 
@@ -123,13 +134,25 @@ def _sig(lib):
         "tdbp_pair_count": (C.c_int, [_GP, _VP, _VP, _VP]),
         "tdbp_pack_samples": (C.c_int, [_VP, C.c_int, C.c_int, C.c_int, _VP, _VP, _VP]),
         "tdbp_chirp_decode": (C.c_int, [_VP, C.POINTER(Chirp)]),
+        "tdbp_burst_decode": (C.c_int, [_VP, C.POINTER(BurstParams)]),
+        "tdbp_steering_fit": (C.c_int, [_VP, C.c_size_t, C.c_int, _DP, _DP]),
+        "tdbp_aperture": (C.c_int, [_DP, C.c_double, C.c_double, C.c_double, C.c_double]),
+        "tdbp_bounds": (C.c_int, [C.c_double, C.c_double, C.c_double, C.c_double, _DP, _DP,
+                                  _DP, C.c_int, _DP]),
+        "tdbp_headers_dtoh": (C.c_int, [_VP, C.c_int, C.c_int, _VP, _VP]),
+        "tdbp_burst_setup": (C.c_int, [_VP, C.c_size_t, C.c_int, C.c_int, _DP, _DP, _DP, _DP,
+                                       C.c_int, C.c_int, C.c_int, C.c_double, C.c_double,
+                                       C.c_double, C.c_double, _GP, _DP, _IP]),
+        "tdbp_pulse_orbit": (C.c_int, [_VP, C.c_int, C.c_int, _DP, _DP, _DP, C.c_int, C.c_int,
+                                       _VP, _VP, _VP, _VP]),
         "tdbp_crossmul": (C.c_int, [_VP, _VP, C.c_int, C.c_int, C.c_float, C.c_int, C.c_int,
                                     C.c_int, _VP, _VP, _VP]),
         "tdbp_makecc": (C.c_int, [_VP, _VP, C.c_int, C.c_int, _VP, _VP]),
         "tdbp_synth_raw": (C.c_int, [_VP, C.c_int, C.c_int, C.POINTER(Chirp), _VP, _DP, _DP,
                                      _VP, C.c_int, C.c_float, C.c_uint64, _VP]),
         "tdbp_range_compress": (C.c_int, [_VP, C.c_int, C.c_int, C.POINTER(Chirp), _VP,
-                                          C.c_int, _VP, _VP, C.c_int, _VP]),
+                                          C.c_int, C.c_int, _VP, _VP, C.c_int, _VP]),
+        "tdbp_headers_htod": (C.c_int, [_VP, C.c_int, C.c_int, _VP, _VP]),
         "tdbp_pixelint": (C.c_int, [_GP, _VP, _VP, _VP, _VP, _VP, _VP]),
         "tdbp_backproject": (C.c_int, [_GP, _SP, _VP, _VP, _VP, _VP, C.c_int, _VP]),
         "tdbp_backproject_variant": (C.c_int, [_GP, _SP, _VP, _VP, _VP, _VP, C.c_int,
@@ -176,7 +199,8 @@ def exported_symbols():
         "tdbp_event_record", "tdbp_event_elapsed_ms", "tdbp_mem_info", "tdbp_window",
         "tdbp_orbit_interp", "tdbp_setxyz", "tdbp_fit_aperture", "tdbp_setazoff",
         "tdbp_pair_count", "tdbp_pack_samples", "tdbp_chirp_decode", "tdbp_range_compress", "tdbp_synth_raw", "tdbp_crossmul",
-        "tdbp_makecc", "tdbp_pixelint", "tdbp_backproject",
+        "tdbp_makecc", "tdbp_burst_decode", "tdbp_steering_fit", "tdbp_aperture", "tdbp_bounds",
+        "tdbp_pulse_orbit", "tdbp_headers_dtoh", "tdbp_burst_setup", "tdbp_headers_htod", "tdbp_pixelint", "tdbp_backproject",
         "tdbp_backproject_variant", "tdbp_autotune", "tdbp_set_variant",
         "tdbp_variant_count", "tdbp_variant_info", "tdbp_batch_create", "tdbp_batch_add",
         "tdbp_batch_run", "tdbp_batch_set_merge", "tdbp_merge_geo", "tdbp_compare", "tdbp_dropin_timing", "tdbp_batch_pairs",
@@ -471,11 +495,108 @@ def chirp_decode(header: bytes) -> Chirp:
     return c
 
 
+def burst_decode(header: bytes) -> BurstParams:
+    """Burst parameters from a line's 80-byte header (host code)."""
+    h = np.frombuffer(bytes(header), np.uint8)
+    p = BurstParams()
+    check(lib().tdbp_burst_decode(h.ctypes.data, C.byref(p)))
+    return p
+
+
+def steering_fit(headers: np.ndarray):
+    """(angc0, angc1) from the headers of a burst's lines: uint8 [lines][>= 80]."""
+    h = np.ascontiguousarray(headers, np.uint8)
+    c0, c1 = C.c_double(), C.c_double()
+    check(lib().tdbp_steering_fit(h.ctypes.data, h.shape[1], h.shape[0], C.byref(c0),
+                                  C.byref(c1)))
+    return c0.value, c1.value
+
+
+def aperture(vel_mid_burst, range0: float, wvl: float, prf: float, angc1: float) -> int:
+    v = np.ascontiguousarray(vel_mid_burst, np.float64)
+    return int(lib().tdbp_aperture(v.ctypes.data_as(_DP), range0, wvl, prf, angc1))
+
+
+def bounds(startime, endtime, startr, endr, timeorbit, xx, vv) -> np.ndarray:
+    """[min lat, max lat, min lon, max lon] (deg) of a burst footprint."""
+    t = np.ascontiguousarray(timeorbit, np.float64)
+    x = np.ascontiguousarray(xx, np.float64).reshape(-1, 3)
+    v = np.ascontiguousarray(vv, np.float64).reshape(-1, 3)
+    out = np.zeros(4)
+    dp = lambda a: a.ctypes.data_as(_DP)  # noqa: E731
+    check(lib().tdbp_bounds(startime, endtime, startr, endr, dp(t), dp(x), dp(v), len(t),
+                            dp(out)))
+    return out
+
+
+def headers_dtoh(d_lines: DeviceArray, stream=None) -> np.ndarray:
+    """uint8 [nlines][80]: each decoded line's packet header."""
+    nl, stride = d_lines.shape
+    out = np.empty((nl, 80), np.uint8)
+    check(lib().tdbp_headers_dtoh(d_lines.ptr, nl, stride, out.ctypes.data, _s(stream)))
+    return out
+
+
+def headers_htod(d_lines: DeviceArray, headers: np.ndarray, stream=None):
+    """Write [nlines][80] header bytes into the device lines."""
+    nl, stride = d_lines.shape
+    h = np.ascontiguousarray(headers, np.uint8)
+    if h.shape != (nl, 80):
+        raise ValueError(f"headers {h.shape} for {nl} lines")
+    check(lib().tdbp_headers_htod(d_lines.ptr, nl, stride, h.ctypes.data, _s(stream)))
+
+
+def burst_setup(headers: np.ndarray, rangesamples: int, satvel_mid, timeorbit, xx, vv,
+                demwidth: int, demlength: int, firstlat: float, firstlon: float,
+                deltalat: float, deltalon: float):
+    """(BurstGeometry, latlons, in_scene) of the driver's per-burst set-up."""
+    h = np.ascontiguousarray(headers, np.uint8)
+    t = np.ascontiguousarray(timeorbit, np.float64)
+    x = np.ascontiguousarray(xx, np.float64).reshape(-1, 3)
+    v = np.ascontiguousarray(vv, np.float64).reshape(-1, 3)
+    sv = np.ascontiguousarray(satvel_mid, np.float64)
+    g = Geom()
+    ll = np.zeros(4)
+    ins = C.c_int()
+    dp = lambda a: a.ctypes.data_as(_DP)  # noqa: E731
+    check(lib().tdbp_burst_setup(h.ctypes.data, h.shape[1], h.shape[0], rangesamples, dp(sv),
+                                 dp(t), dp(x), dp(v), len(t), demwidth, demlength, firstlat,
+                                 firstlon, deltalat, deltalon, C.byref(g), dp(ll),
+                                 C.byref(ins)))
+    geom = BurstGeometry(
+        L=g.rawdatalines, S=g.samples, A=g.aperture, rngstart=g.rngstart, rngend=g.rngend,
+        dmrg=g.dmrg, wvl=g.wvl, prf=g.prf, angc0=g.angc0, angc1=g.angc1, tmid=g.tmid,
+        xyz_mid=np.array(g.xyz_mid[:]), vel_mid=np.array(g.vel_mid[:]), timeorbit=t, xx=x, vv=v,
+        demwidth=g.demwidth, firstline=g.firstline, nlines=g.nlines, firstpix=g.firstpix,
+        lastpix=g.lastpix, firstlat=g.firstlat, firstlon=g.firstlon, deltalat=g.deltalat,
+        deltalon=g.deltalon, fit_last_line=g.fit_last_line)
+    return geom, ll, bool(ins.value)
+
+
+def pulse_orbit(d_lines: DeviceArray, timeorbit, xx, vv, quantize: bool = False, stream=None):
+    """Per-line (satloc, satvel, times) device arrays from the header times."""
+    nl, stride = d_lines.shape
+    t = np.ascontiguousarray(timeorbit, np.float64)
+    x = np.ascontiguousarray(xx, np.float64).reshape(-1, 3)
+    v = np.ascontiguousarray(vv, np.float64).reshape(-1, 3)
+    sat = DeviceArray((nl, 3), np.float64)
+    vel = DeviceArray((nl, 3), np.float64)
+    tim = DeviceArray((nl,), np.float64)
+    dp = lambda a: a.ctypes.data_as(_DP)  # noqa: E731
+    check(lib().tdbp_pulse_orbit(d_lines.ptr, nl, stride, dp(t), dp(x), dp(v), len(t),
+                                 1 if quantize else 0, sat.ptr, vel.ptr, tim.ptr, _s(stream)))
+    return sat, vel, tim
+
+
+RC_ZERO_HEADER = 1
+
+
 def range_compress(d_raw: DeviceArray, chirp: Chirp, d_out: Optional[DeviceArray] = None,
-                   packed=None, stream=None):
+                   packed=None, stream=None, zero_header: bool = False):
     """Range compression of d_raw (cf32 [nlines][raw_stride], header in the
-    first 10 samples).  d_out: cf32 [nlines][>= nvalid + 10]; packed: (dtype,
-    d_words [nlines][nvalid + 10], d_scale [nlines])."""
+    first 10 samples).  d_out: cf32 [nlines][>= nvalid + 10] (header samples
+    copied, or zero with zero_header); packed: (dtype, d_words [nlines]
+    [nvalid + 10], d_scale [nlines])."""
     nlines, stride = d_raw.shape
     if d_out is not None and (d_out.shape[0] != nlines or d_out.dtype != np.complex64):
         raise ValueError("output must be cf32 with one row per line")
@@ -485,7 +606,8 @@ def range_compress(d_raw: DeviceArray, chirp: Chirp, d_out: Optional[DeviceArray
         raise ValueError("packed output buffers too small")
     check(lib().tdbp_range_compress(
         d_raw.ptr, nlines, stride, C.byref(chirp), d_out.ptr if d_out is not None else None,
-        d_out.shape[1] if d_out is not None else 0, pw.ptr if pw is not None else None,
+        d_out.shape[1] if d_out is not None else 0, RC_ZERO_HEADER if zero_header else 0,
+        pw.ptr if pw is not None else None,
         ps.ptr if ps is not None else None, dt, _s(stream)))
 
 

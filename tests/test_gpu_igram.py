@@ -44,6 +44,20 @@ def test_crossmul_matches_oracle(gpu, images, looks):
     assert rel(gi, oi) < 1e-5 and rel(ga, oa) < 1e-5, (rel(gi, oi), rel(ga, oa))
 
 
+def test_crossmul_narrow_after_wide(gpu):
+    """A narrower image after a wider one in the same process: the transform
+    length is the narrow image's own (next power of two >= its width), not
+    that of the buffers left by the wider call."""
+    rng = np.random.default_rng(12)
+    for w in (4000, 1500, 700):
+        a = (rng.standard_normal((40, w)) + 1j * rng.standard_normal((40, w))).astype(np.complex64)
+        b = (rng.standard_normal((40, w)) + 1j * rng.standard_normal((40, w))).astype(np.complex64)
+        ig, am = T.crossmul(T.DeviceArray.from_host(a), T.DeviceArray.from_host(b), looksac=2)
+        T.sync()
+        oi, oa = IO.crossmul(a, b, looksac=2)
+        assert rel(ig.download(), oi) < 1e-5 and rel(am.download(), oa) < 1e-5, w
+
+
 def test_image_with_itself_and_makecc(gpu, images):
     a, _ = images
     d = T.DeviceArray.from_host(a)
