@@ -400,6 +400,32 @@ def stack_bench(args, synth, T):
     amp = [e["amp_rms"] for e in per_acq]
     log(f"per-acquisition phase vs FP64 (mrad): max {max(ph):.4f} mean {np.mean(ph):.4f}; "
         f"amp max {max(amp):.2e}")
+
+    # Interferograms of consecutive acquisitions on the merged grids, where the
+    # stack left them (int/crossmul.f90 with its defaults: scale 1, mask on).
+    igram = None
+    if nacq > 1 and args.igram_looks > 0:
+        lk = args.igram_looks
+        T.sync()
+        ev_i0, ev_i1 = T.Event(), T.Event()
+        ig_ms = []
+        for a in range(nacq - 1):
+            ev_i0.record(stream)
+            ig, am = T.crossmul(g_mixed[a], g_mixed[a + 1], looksac=lk, looksdn=lk,
+                                stream=stream)
+            ev_i1.record(stream)
+            stream.sync()
+            ig_ms.append(ev_i0.elapsed_ms(ev_i1))
+            ig.free()
+            am.free()
+        ms_pair = float(np.median(ig_ms[1:] if len(ig_ms) > 1 else ig_ms))
+        px = grid.lines * grid.W
+        igram = {"pairs": nacq - 1, "looks": [lk, lk], "ms_per_interferogram": round(ms_pair, 3),
+                 "input_gpx_per_s": round(px / ms_pair / 1e6, 2),
+                 "hbm_gbps_algorithmic": round((2 * px * 8 + 2 * px * 8 / lk / lk) / ms_pair / 1e6,
+                                               1)}
+        log(f"interferograms: {ms_pair:.2f} ms per {grid.lines} x {grid.W} pair "
+            f"(looks {lk}x{lk})")
     line = {
         "metric": METRIC, "value": round(gpps, 2), "unit": "Gpixel·pulse/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(step_ms, 3),
@@ -421,6 +447,7 @@ def stack_bench(args, synth, T):
             "phase_mrad_max": round(max(ph), 5), "amp_rel_max": max(amp),
             "extra_nonzero_pixels": sum(e["extra_nonzero"] for e in per_acq)},
         "baselines_m": [round(float(x), 2) for x in np.linalg.norm(stk.baselines, axis=1)],
+        "interferograms": igram,
     }
     print(json.dumps(line), flush=True)
 
@@ -443,6 +470,8 @@ def main():
                          "dropin: azimuth_compress_ end to end on one IW burst; rc: range "
                          "compression of one IW burst and raw -> image")
     ap.add_argument("--acqs", type=int, default=30, help="stack: acquisitions")
+    ap.add_argument("--igram-looks", type=int, default=1,
+                    help="stack: looks of the consecutive-pair interferograms (0: none)")
     ap.add_argument("--hbm-budget", type=float, default=0.0,
                     help="stack: admission budget in GiB (default: free HBM less a reserve)")
     ap.add_argument("--profile", action="store_true",
