@@ -106,8 +106,9 @@ __global__ void k_rc_mul(float2* __restrict__ buf, const float2* __restrict__ re
   *p = make_float2(a.x * r.x + a.y * r.y, a.y * r.x - a.x * r.y);
 }
 
-// Output stage, one block per line.
-constexpr int kStoreBlock = 256;
+// Output stage, one block of 16 waves per line (a batch is 256 lines: one
+// block per CU would leave most of each CU idle).
+constexpr int kStoreBlock = 1024;
 __global__ void __launch_bounds__(kStoreBlock)
 k_rc_store(const float2* __restrict__ raw, int nlines, int stride, int base,
            const float2* __restrict__ buf, int nvalid, float2* __restrict__ out,
