@@ -100,6 +100,14 @@ void azimuth_compress_(void *burstdata, double *satloc, int *rawdatalines,
  * entry returns nothing). */
 int tdbp_dropin_status(void);
 
+/* With TDBP_DROPIN_ASYNC=1, azimuth_compress_ returns once the burst is on the
+ * device and its sum is enqueued; the .geo read-merge-write of each burst
+ * runs on a worker thread in call order, overlapping the next call's upload.
+ * tdbp_dropin_flush waits until every queued burst is written and returns the
+ * first error among them (0 = all written), then clears it.  The queue is also
+ * drained when the process exits.  No reference counterpart. */
+int tdbp_dropin_flush(void);
+
 /* Phase times (ms) of the last azimuth_compress_ call: [0] total, [1] DEM
  * read, [2] DEM + satloc + burst upload, [3] device compute (pack, fit,
  * sum), [4] .geo read (overlaps [3]), [5] .geo upload + merge + download,

@@ -8,7 +8,11 @@
 //   * the .geo rows [firstline, firstline+nlines) read back from fdout,
 //     pixels replaced where |Re(out)| > 1e-18, written back (:513-537).
 // The DEM and .geo rows pass through pinned staging buffers; the .geo rows
-// are read while the device integrates and merged on the device.
+// are read while the device integrates and merged on the device.  With
+// TDBP_DROPIN_ASYNC=1 the read-merge-write runs on a worker thread after the
+// call returns (in call order; all of it done before the process exits, or at
+// tdbp_dropin_flush) -- the reference's own .geo FILE* is not flushed before
+// exit either (sentinel/azimuth_compress.cu:518).
 // Differences, all deliberate: pread/pwrite instead of a never-flushed FILE*
 // (the bytes land at once and the fd offsets are untouched); a short DEM
 // read is an error (the .geo is left unchanged) instead of integrating over
@@ -21,8 +25,11 @@
 #include <cstdio>
 #include <cstdlib>
 #include <chrono>
+#include <condition_variable>
 #include <cstring>
+#include <deque>
 #include <mutex>
+#include <thread>
 #include <vector>
 
 #include "tdbp_api_internal.h"
@@ -32,56 +39,80 @@ using namespace tdbp;
 
 namespace {
 
-// Grow-only device buffers and pinned host staging reused by every call in
-// the process.
+// Two slots of grow-only device buffers and pinned host staging, reused by
+// every call in the process.  A synchronous call uses slot 0.  With
+// TDBP_DROPIN_ASYNC=1 calls alternate between the slots: a call returns once
+// its burst is on the device and its sum is enqueued, and a worker thread
+// does that burst's .geo read-merge-write, in call order, while the next call
+// uploads into the other slot.
 enum { kBurst, kPacked, kScale, kSat, kDem, kCoef, kOut, kGeo, kNumDev };
 enum { kHostDem, kHostGeo, kNumHost };
-struct Pool {
+struct Slot {
   void* p[kNumDev] = {};
   size_t cap[kNumDev] = {};
   void* h[kNumHost] = {};
   size_t hcap[kNumHost] = {};
   hipStream_t s = nullptr;
   hipEvent_t ev[6] = {};
+  bool busy = false;   // an asynchronous .geo job still uses this slot
 };
 
-std::mutex g_mu;
-Pool g_pool;
+// One pending .geo read-merge-write.
+struct GeoJob {
+  int slot;
+  int fdout;
+  off_t off;
+  size_t npx;
+  int firstline;
+};
+
+std::mutex g_mu;                 // one call at a time, as the reference driver makes them
+Slot g_slot[2];
+int g_next = 0;
 thread_local int g_status = TDBP_OK;
 // Phase times of the last call, ms: total, DEM read, burst+satloc+DEM upload,
 // device compute (pack, fit, sum), .geo read, .geo upload+merge+download,
-// .geo write.
+// .geo write (the last three of a synchronous call only).
 double g_timing[7] = {};
 
-int need(int slot, size_t bytes) {
-  if (g_pool.cap[slot] >= bytes && g_pool.p[slot]) return TDBP_OK;
-  if (g_pool.p[slot]) {
-    (void)hipStreamSynchronize(g_pool.s);
-    (void)hipFree(g_pool.p[slot]);
-    g_pool.p[slot] = nullptr;
-    g_pool.cap[slot] = 0;
+// Worker state (guarded by g_q_mu).
+std::mutex g_q_mu;
+std::condition_variable g_q_cv;
+std::deque<GeoJob> g_q;
+std::thread g_worker;
+bool g_worker_up = false, g_stop = false;
+int g_async_status = TDBP_OK;
+char g_async_msg[256] = "";
+
+int need(Slot& sl, int k, size_t bytes) {
+  if (sl.cap[k] >= bytes && sl.p[k]) return TDBP_OK;
+  if (sl.p[k]) {
+    (void)hipStreamSynchronize(sl.s);
+    (void)hipFree(sl.p[k]);
+    sl.p[k] = nullptr;
+    sl.cap[k] = 0;
   }
-  if (hipMalloc(&g_pool.p[slot], bytes) != hipSuccess) {
-    g_pool.p[slot] = nullptr;
+  if (hipMalloc(&sl.p[k], bytes) != hipSuccess) {
+    sl.p[k] = nullptr;
     return set_error(TDBP_ERR_NOMEM, "drop-in hipMalloc(%zu) failed", bytes);
   }
-  g_pool.cap[slot] = bytes;
+  sl.cap[k] = bytes;
   return TDBP_OK;
 }
 
-int need_host(int slot, size_t bytes) {
-  if (g_pool.hcap[slot] >= bytes && g_pool.h[slot]) return TDBP_OK;
-  if (g_pool.h[slot]) {
-    (void)hipStreamSynchronize(g_pool.s);
-    (void)hipHostFree(g_pool.h[slot]);
-    g_pool.h[slot] = nullptr;
-    g_pool.hcap[slot] = 0;
+int need_host(Slot& sl, int k, size_t bytes) {
+  if (sl.hcap[k] >= bytes && sl.h[k]) return TDBP_OK;
+  if (sl.h[k]) {
+    (void)hipStreamSynchronize(sl.s);
+    (void)hipHostFree(sl.h[k]);
+    sl.h[k] = nullptr;
+    sl.hcap[k] = 0;
   }
-  if (hipHostMalloc(&g_pool.h[slot], bytes, hipHostMallocDefault) != hipSuccess) {
-    g_pool.h[slot] = nullptr;
+  if (hipHostMalloc(&sl.h[k], bytes, hipHostMallocDefault) != hipSuccess) {
+    sl.h[k] = nullptr;
     return set_error(TDBP_ERR_NOMEM, "drop-in hipHostMalloc(%zu) failed", bytes);
   }
-  g_pool.hcap[slot] = bytes;
+  sl.hcap[k] = bytes;
   return TDBP_OK;
 }
 
@@ -118,6 +149,122 @@ ssize_t read_full(int fd, void* buf, size_t n, off_t off) {
     got += (size_t)r;
   }
   return (ssize_t)got;
+}
+
+bool write_full(int fd, const void* buf, size_t n, off_t off) {
+  size_t put = 0;
+  while (put < n) {
+    const ssize_t w = pwrite(fd, (const char*)buf + put, n - put, off + (off_t)put);
+    if (w <= 0) return false;
+    put += (size_t)w;
+  }
+  return true;
+}
+
+// The .geo read-merge-write of one burst whose sum is enqueued on the slot's
+// stream: rows read into pinned memory (overlapping the sum), merged on the
+// device, written back.  times (optional): read, merge round trip, write ms.
+int geo_rmw(Slot& sl, int fdout, off_t off, size_t npx, int firstline, double* times,
+            char* msg, size_t msglen) {
+  float2* hgeo = (float2*)sl.h[kHostGeo];
+  const double t2 = now_ms();
+  const ssize_t gg = read_full(fdout, hgeo, npx * 8, off);
+  if (gg < 0) {
+    (void)hipStreamSynchronize(sl.s);
+    std::snprintf(msg, msglen, ".geo read at line %d failed", firstline);
+    return TDBP_ERR_IO;
+  }
+  if ((size_t)gg < npx * 8)  // rows past the end of the file read as zero
+    std::memset((char*)hgeo + gg, 0, npx * 8 - (size_t)gg);
+  const double t3 = now_ms();
+  (void)hipEventRecord(sl.ev[3], sl.s);
+  if (hipMemcpyAsync(sl.p[kGeo], hgeo, npx * 8, hipMemcpyHostToDevice, sl.s) != hipSuccess ||
+      launch_merge((float2*)sl.p[kGeo], (const float2*)sl.p[kOut], npx, 0, sl.s) !=
+          hipSuccess ||
+      hipMemcpyAsync(hgeo, sl.p[kGeo], npx * 8, hipMemcpyDeviceToHost, sl.s) != hipSuccess) {
+    (void)hipStreamSynchronize(sl.s);
+    std::snprintf(msg, msglen, ".geo merge at line %d failed", firstline);
+    return TDBP_ERR_HIP;
+  }
+  (void)hipEventRecord(sl.ev[4], sl.s);
+  if (hipStreamSynchronize(sl.s) != hipSuccess) {
+    std::snprintf(msg, msglen, "drop-in device work failed (line %d)", firstline);
+    return TDBP_ERR_HIP;
+  }
+  const double t4 = now_ms();
+  if (!write_full(fdout, hgeo, npx * 8, off)) {
+    std::snprintf(msg, msglen, ".geo write at line %d failed", firstline);
+    return TDBP_ERR_IO;
+  }
+  if (times) {
+    float rmw = 0.f;
+    (void)hipEventElapsedTime(&rmw, sl.ev[3], sl.ev[4]);
+    times[0] = t3 - t2;
+    times[1] = rmw;
+    times[2] = now_ms() - t4;
+  }
+  return TDBP_OK;
+}
+
+void worker_main() {
+  (void)hipSetDevice(0);
+  for (;;) {
+    GeoJob j;
+    {
+      std::unique_lock<std::mutex> lk(g_q_mu);
+      g_q_cv.wait(lk, [] { return g_stop || !g_q.empty(); });
+      if (g_q.empty()) return;   // stop requested and nothing left
+      j = g_q.front();
+    }
+    char msg[256] = "";
+    const int st = geo_rmw(g_slot[j.slot], j.fdout, j.off, j.npx, j.firstline, nullptr, msg,
+                           sizeof msg);
+    if (st != TDBP_OK)
+      std::fprintf(stderr, "azimuth_compress: burst .geo not updated: %s\n", msg);
+    {
+      std::lock_guard<std::mutex> lk(g_q_mu);
+      g_q.pop_front();
+      g_slot[j.slot].busy = false;
+      if (st != TDBP_OK && g_async_status == TDBP_OK) {
+        g_async_status = st;
+        std::snprintf(g_async_msg, sizeof g_async_msg, "%s", msg);
+      }
+    }
+    g_q_cv.notify_all();
+  }
+}
+
+// Waits until every queued .geo job is written.
+void drain() {
+  std::unique_lock<std::mutex> lk(g_q_mu);
+  g_q_cv.wait(lk, [] { return g_q.empty(); });
+}
+
+void stop_worker() {
+  {
+    std::unique_lock<std::mutex> lk(g_q_mu);
+    if (!g_worker_up) return;
+    g_q_cv.wait(lk, [] { return g_q.empty(); });
+    g_stop = true;
+  }
+  g_q_cv.notify_all();
+  g_worker.join();
+  std::lock_guard<std::mutex> lk(g_q_mu);
+  g_worker_up = false;
+  g_stop = false;
+}
+
+void start_worker() {
+  std::lock_guard<std::mutex> lk(g_q_mu);
+  if (g_worker_up) return;
+  static bool registered = false;
+  if (!registered) {
+    // The .geo bytes of every burst land before the process ends.
+    std::atexit(stop_worker);
+    registered = true;
+  }
+  g_worker = std::thread(worker_main);
+  g_worker_up = true;
 }
 
 int run(const float* burst, const double* satloc, int L, int S, int W,
@@ -174,29 +321,38 @@ int run(const float* burst, const double* satloc, int L, int S, int W,
   int prec, dtype;
   precision_from_env(&prec, &dtype);
 
+  const char* ea = std::getenv("TDBP_DROPIN_ASYNC");
+  const bool async = ea && ea[0] == '1';
+
   std::lock_guard<std::mutex> lk(g_mu);
   const double t0 = now_ms();
-  if (!g_pool.s) {
-    if ((st = hip_check(hipStreamCreateWithFlags(&g_pool.s, hipStreamNonBlocking),
+  if (!async) drain();  // queued bursts first: their rows may overlap this one's
+  const int si = async ? g_next : 0;
+  Slot& sl = g_slot[si];
+  {
+    std::unique_lock<std::mutex> ql(g_q_mu);
+    g_q_cv.wait(ql, [&] { return !sl.busy; });
+  }
+  if (!sl.s) {
+    if ((st = hip_check(hipStreamCreateWithFlags(&sl.s, hipStreamNonBlocking),
                         "drop-in stream")))
       return st;
-    for (auto& e : g_pool.ev)
+    for (auto& e : sl.ev)
       if ((st = hip_check(hipEventCreate(&e), "drop-in event"))) return st;
   }
-  hipStream_t s = g_pool.s;
-  hipEvent_t* ev = g_pool.ev;
+  hipStream_t s = sl.s;
+  hipEvent_t* ev = sl.ev;
   const size_t npx = (size_t)nlines * W;
   const size_t nsamp = (size_t)L * S;
-  if ((st = need(kBurst, nsamp * 8)) || (st = need(kSat, (size_t)L * 24)) ||
-      (st = need(kDem, npx * 2)) || (st = need(kCoef, (size_t)nlines * 24)) ||
-      (st = need(kOut, npx * 8)) || (st = need(kGeo, npx * 8)) ||
-      (st = need_host(kHostDem, npx * 2)) || (st = need_host(kHostGeo, npx * 8)))
+  if ((st = need(sl, kBurst, nsamp * 8)) || (st = need(sl, kSat, (size_t)L * 24)) ||
+      (st = need(sl, kDem, npx * 2)) || (st = need(sl, kCoef, (size_t)nlines * 24)) ||
+      (st = need(sl, kOut, npx * 8)) || (st = need(sl, kGeo, npx * 8)) ||
+      (st = need_host(sl, kHostDem, npx * 2)) || (st = need_host(sl, kHostGeo, npx * 8)))
     return st;
   if (dtype != TDBP_CF32 &&
-      ((st = need(kPacked, nsamp * 4)) || (st = need(kScale, (size_t)L * 4))))
+      ((st = need(sl, kPacked, nsamp * 4)) || (st = need(sl, kScale, (size_t)L * 4))))
     return st;
-  int16_t* hdem = (int16_t*)g_pool.h[kHostDem];
-  float2* hgeo = (float2*)g_pool.h[kHostGeo];
+  int16_t* hdem = (int16_t*)sl.h[kHostDem];
 
   // DEM rows into pinned memory, then everything the device needs.
   const ssize_t got = read_full(fddem, hdem, npx * 2, (off_t)firstline * W * 2);
@@ -205,84 +361,67 @@ int run(const float* burst, const double* satloc, int L, int S, int W,
                      npx * 2, firstline, got);
   const double t1 = now_ms();
   (void)hipEventRecord(ev[0], s);
-  if ((st = hip_check(hipMemcpyAsync(g_pool.p[kDem], hdem, npx * 2,
-                                     hipMemcpyHostToDevice, s),
+  if ((st = hip_check(hipMemcpyAsync(sl.p[kDem], hdem, npx * 2, hipMemcpyHostToDevice, s),
                       "dem upload")) ||
-      (st = hip_check(hipMemcpyAsync(g_pool.p[kSat], satloc, (size_t)L * 24,
+      (st = hip_check(hipMemcpyAsync(sl.p[kSat], satloc, (size_t)L * 24,
                                      hipMemcpyHostToDevice, s),
                       "satloc upload")) ||
-      (st = hip_check(hipMemcpyAsync(g_pool.p[kBurst], burst, nsamp * 8,
-                                     hipMemcpyHostToDevice, s),
+      (st = hip_check(hipMemcpyAsync(sl.p[kBurst], burst, nsamp * 8, hipMemcpyHostToDevice,
+                                     s),
                       "burst upload")))
     return st;
   (void)hipEventRecord(ev[1], s);
   tdbp_samples smp;
-  smp.d_data = g_pool.p[kBurst];
+  smp.d_data = sl.p[kBurst];
   smp.d_scale = nullptr;
   smp.dtype = TDBP_CF32;
   if (dtype != TDBP_CF32) {
-    if ((st = tdbp_pack_samples(g_pool.p[kBurst], L, S, dtype, g_pool.p[kPacked],
-                                (float*)g_pool.p[kScale], s)))
+    if ((st = tdbp_pack_samples(sl.p[kBurst], L, S, dtype, sl.p[kPacked],
+                                (float*)sl.p[kScale], s)))
       return st;
-    smp.d_data = g_pool.p[kPacked];
-    smp.d_scale = (const float*)g_pool.p[kScale];
+    smp.d_data = sl.p[kPacked];
+    smp.d_scale = (const float*)sl.p[kScale];
     smp.dtype = dtype;
   }
-  if ((st = tdbp_fit_aperture(&g, (const int16_t*)g_pool.p[kDem],
-                              (double*)g_pool.p[kCoef], s)))
+  if ((st = tdbp_fit_aperture(&g, (const int16_t*)sl.p[kDem], (double*)sl.p[kCoef], s)))
     return st;
-  if ((st = run_backproject(&g, &smp, (const double*)g_pool.p[kSat],
-                            (const int16_t*)g_pool.p[kDem],
-                            (const double*)g_pool.p[kCoef], g_pool.p[kOut], prec,
-                            -1, s)))
+  if ((st = run_backproject(&g, &smp, (const double*)sl.p[kSat], (const int16_t*)sl.p[kDem],
+                            (const double*)sl.p[kCoef], sl.p[kOut], prec, -1, s)))
     return st;
   (void)hipEventRecord(ev[2], s);
-
-  // The .geo rows are read while the device integrates; they are merged on
-  // the device and come back in the same pinned buffer.
   const off_t off = (off_t)firstline * W * 8;
-  const double t2 = now_ms();
-  const ssize_t gg = read_full(fdout, hgeo, npx * 8, off);
-  if (gg < 0) {
-    (void)hipStreamSynchronize(s);
-    return set_error(TDBP_ERR_IO, ".geo read at line %d failed", firstline);
+
+  if (async) {
+    // The burst is on the device (the pageable upload has returned) and its
+    // sum is enqueued; the worker finishes the .geo rows in call order.
+    start_worker();
+    {
+      std::lock_guard<std::mutex> ql(g_q_mu);
+      sl.busy = true;
+      g_q.push_back(GeoJob{si, fdout, off, npx, firstline});
+    }
+    g_q_cv.notify_all();
+    g_next ^= 1;
+    g_timing[0] = now_ms() - t0;
+    g_timing[1] = t1 - t0;
+    for (int k = 2; k < 7; ++k) g_timing[k] = 0.0;
+    return TDBP_OK;
   }
-  if ((size_t)gg < npx * 8)  // rows past the end of the file read as zero
-    std::memset((char*)hgeo + gg, 0, npx * 8 - (size_t)gg);
-  const double t3 = now_ms();
-  (void)hipEventRecord(ev[3], s);
-  if ((st = hip_check(hipMemcpyAsync(g_pool.p[kGeo], hgeo, npx * 8,
-                                     hipMemcpyHostToDevice, s),
-                      ".geo upload")) ||
-      (st = hip_check(launch_merge((float2*)g_pool.p[kGeo], (const float2*)g_pool.p[kOut],
-                                   npx, 0, s),
-                      "merge launch")) ||
-      (st = hip_check(hipMemcpyAsync(hgeo, g_pool.p[kGeo], npx * 8,
-                                     hipMemcpyDeviceToHost, s),
-                      ".geo download")))
-    return st;
-  (void)hipEventRecord(ev[4], s);
-  if ((st = hip_check(hipStreamSynchronize(s), "drop-in sync"))) return st;
-  const double t4 = now_ms();
-  size_t put = 0;
-  while (put < npx * 8) {
-    const ssize_t w = pwrite(fdout, (const char*)hgeo + put, npx * 8 - put,
-                             off + (off_t)put);
-    if (w <= 0) return set_error(TDBP_ERR_IO, ".geo write failed");
-    put += (size_t)w;
-  }
-  const double t5 = now_ms();
-  float up = 0.f, comp = 0.f, rmw = 0.f;
+
+  double rt[3] = {0.0, 0.0, 0.0};
+  char msg[256] = "";
+  if ((st = geo_rmw(sl, fdout, off, npx, firstline, rt, msg, sizeof msg)))
+    return set_error(st, "%s", msg);
+  float up = 0.f, comp = 0.f;
   (void)hipEventElapsedTime(&up, ev[0], ev[1]);
   (void)hipEventElapsedTime(&comp, ev[1], ev[2]);
-  (void)hipEventElapsedTime(&rmw, ev[3], ev[4]);
-  g_timing[0] = t5 - t0;
+  g_timing[0] = now_ms() - t0;
   g_timing[1] = t1 - t0;
   g_timing[2] = up;
   g_timing[3] = comp;
-  g_timing[4] = t3 - t2;
-  g_timing[5] = rmw;
-  g_timing[6] = t5 - t4;
+  g_timing[4] = rt[0];
+  g_timing[5] = rt[1];
+  g_timing[6] = rt[2];
   return TDBP_OK;
 }
 
@@ -317,6 +456,16 @@ void azimuth_compress_(void* burstdata, double* satloc, int* rawdatalines,
 }
 
 int tdbp_dropin_status(void) { return g_status; }
+
+int tdbp_dropin_flush(void) {
+  drain();
+  std::lock_guard<std::mutex> lk(g_q_mu);
+  const int st = g_async_status;
+  if (st != TDBP_OK) set_error(st, "%s", g_async_msg);
+  g_async_status = TDBP_OK;
+  g_async_msg[0] = 0;
+  return st;
+}
 
 int tdbp_dropin_timing(double* ms, int n) {
   std::lock_guard<std::mutex> lk(g_mu);

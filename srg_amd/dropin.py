@@ -90,6 +90,12 @@ class BurstCall:
         tdbp.check(tdbp.lib().tdbp_dropin_status())
 
 
+def flush():
+    """Wait until every burst queued by TDBP_DROPIN_ASYNC=1 calls is written to
+    its .geo; raises the first error among them."""
+    tdbp.check(tdbp.lib().tdbp_dropin_flush())
+
+
 TIMING_NAMES = ("total", "dem_read", "upload", "compute", "geo_read", "geo_merge",
                 "geo_write")
 

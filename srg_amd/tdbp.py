@@ -109,6 +109,7 @@ def _sig(lib):
         "tdbp_device_count": (C.c_int, []),
         "tdbp_init": (C.c_int, [C.c_int]),
         "tdbp_dropin_status": (C.c_int, []),
+        "tdbp_dropin_flush": (C.c_int, []),
         "tdbp_dev_alloc": (_VP, [C.c_size_t]),
         "tdbp_dev_free": (C.c_int, [_VP]),
         "tdbp_memcpy_htod": (C.c_int, [_VP, _VP, C.c_size_t, _VP]),
@@ -192,7 +193,7 @@ def _sig(lib):
 def exported_symbols():
     """Names of the entry points the binding declares (all must exist)."""
     return [
-        "tdbp_last_error", "tdbp_device_count", "tdbp_init", "tdbp_dropin_status",
+        "tdbp_last_error", "tdbp_device_count", "tdbp_init", "tdbp_dropin_status", "tdbp_dropin_flush",
         "tdbp_dev_alloc", "tdbp_dev_free", "tdbp_memcpy_htod", "tdbp_memcpy_dtoh",
         "tdbp_memcpy_dtod", "tdbp_dev_zero", "tdbp_dev_copy", "tdbp_stream_create", "tdbp_stream_destroy",
         "tdbp_stream_sync", "tdbp_device_sync", "tdbp_event_create", "tdbp_event_destroy",
