@@ -69,7 +69,7 @@ def cpu_baseline(sc, d_data, budget_s=15.0):
     from oracle import pyoracle
 
     g = sc.geom
-    data = d_data.download()
+    data = (d_data.download() if hasattr(d_data, "download") else d_data).reshape(g.L, g.S)
     orc = pyoracle.Oracle()
     kind = "reference" if pyoracle.ref_available() else "port"
     ref = pyoracle.RefCPU() if kind == "reference" else None
@@ -105,6 +105,42 @@ def cpu_baseline(sc, d_data, budget_s=15.0):
             "sample": f"pixelint of burst 0 ({g.L}x{g.S}), window rows {r0}..{r0 + nrows - 1} "
                       f"x {g.demwidth} px, {pairs} pairs in {dt:.1f} s, gcc -O2 -fopenmp, "
                       f"OMP threads {threads}"}
+
+
+def cpu_baseline_rc(lines, chirp_header, budget_s=10.0):
+    """Range compression by the oracle (numpy FFT in FP64, one thread) on a
+    bounded sample: the given lines, repeated until about budget_s of work.
+    The reference's CPU path (processsub.f90 with FFTW) cannot be built here:
+    FFTW is not installed."""
+    from oracle import range_oracle as RO
+
+    c = RO.chirp_params(chirp_header)
+    done, t0 = 0, time.perf_counter()
+    while time.perf_counter() - t0 < budget_s:
+        RO.compress(lines, c)
+        done += len(lines)
+    dt = time.perf_counter() - t0
+    return {"value": round(done / dt, 2), "unit": "lines/s", "cores": 1, "kind": "port",
+            "sample": f"oracle range compression (numpy FFT, FP64) of {done} lines "
+                      f"({len(lines)} distinct) in {dt:.1f} s; the reference's FFTW path "
+                      f"is not buildable here"}
+
+
+def cpu_baseline_igram(img1, img2, rows_total, budget_s=10.0):
+    """crossmul by the oracle (numpy, FP64, one thread) on a bounded sample
+    of grid rows, repeated until about budget_s of work, scaled to a whole
+    interferogram."""
+    from oracle import igram_oracle as IO
+
+    done, t0 = 0, time.perf_counter()
+    while time.perf_counter() - t0 < budget_s:
+        IO.crossmul(img1, img2)
+        done += img1.shape[0]
+    dt = time.perf_counter() - t0
+    return {"value": round(dt / done * rows_total * 1e3, 1), "unit": "ms per interferogram",
+            "cores": 1, "kind": "port",
+            "sample": f"oracle crossmul (numpy, FP64) of {done} rows x {img1.shape[1]} "
+                      f"({img1.shape[0]} distinct) in {dt:.1f} s, scaled to {rows_total} rows"}
 
 
 def single_burst(args, synth, T):
@@ -169,6 +205,7 @@ def single_burst(args, synth, T):
         log(f"mixed {name}: {ms:.3f} ms ({pairs / ms / 1e6:.1f} GP/s), phase {ph * 1e3:.4f} "
             f"mrad, amp {amp:.2e}")
     best = res["mixed_cf16"]
+    cpu = None if args.no_cpu else cpu_baseline(sc, d_data)
     line = {"metric": METRIC, "value": best["gpps"], "unit": "Gpixel·pulse/s", "n_gpus": 1,
             "steps": args.steps, "warmup": 1, "ms_per_step": best["ms"],
             "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
@@ -177,7 +214,7 @@ def single_burst(args, synth, T):
                 "one IW burst 1500 x 21000 -> 4000 x 1500 px, fBm DEM" if full else
                 "512 x 4096 burst -> 256 x 256 px, flat DEM, A=256"),
                 "pairs": pairs, "errors_against": against},
-            "results": res}
+            "results": res, "cpu_baseline": cpu}
     print(json.dumps(line), flush=True)
 
 
@@ -245,6 +282,7 @@ def dropin_bench(args, synth, T):
             os.close(fdd)
             os.close(fdo)
     best = res["mixed_fp16"]
+    cpu = None if args.no_cpu else cpu_baseline(sc, data)
     line = {"metric": METRIC + " (drop-in, end to end)", "value": best["gpps_end_to_end"],
             "unit": "Gpixel·pulse/s", "n_gpus": 1, "steps": args.steps,
             "warmup": args.warmup, "ms_per_step": best["ms_per_call"],
@@ -253,7 +291,7 @@ def dropin_bench(args, synth, T):
             "config": {"workload": "azimuth_compress_ on one IW burst 1500 x 21000 -> "
                                    "4000 x 1500 px, host buffers, DEM/.geo files",
                        "pairs": pairs},
-            "results": res}
+            "results": res, "cpu_baseline": cpu}
     print(json.dumps(line), flush=True)
 
 
@@ -312,6 +350,7 @@ def rc_bench(args, synth, T):
     img = out.download()
     foc = [abs(np.angle(img[r, col] * np.conj(a))) for (r, col), a in
            zip(sc.targets_px, sc.targets_amp)]
+    cpu = None if args.no_cpu else cpu_baseline_rc(d_raw.rows(0, 256).download(), h)
     log(f"range compression (fused cf16): {ms_rc:.3f} ms/burst ({g.L / ms_rc * 1e3:.0f} lines/s, "
         f"{gbps:.0f} GB/s algorithmic); with cf32 copy {ms_both:.3f} ms; backprojection "
         f"{ms_bp:.3f} ms; raw -> image {ms_e2e:.3f} ms; target phase errors (rad) "
@@ -331,7 +370,8 @@ def rc_bench(args, synth, T):
             "ms_with_cf32_copy": round(ms_both, 4), "ms_backprojection": round(ms_bp, 4),
             "ms_raw_to_image": round(ms_e2e, 4),
             "raw_to_image_gpps": round(pairs / ms_e2e / 1e6, 2),
-            "target_phase_err_rad": [round(float(x), 5) for x in foc]}
+            "target_phase_err_rad": [round(float(x), 5) for x in foc],
+            "cpu_baseline": cpu}
     print(json.dumps(line), flush=True)
 
 
@@ -365,12 +405,17 @@ def stack_bench(args, synth, T):
     e0, e1 = T.Event(), T.Event()
     plain = {"ms": 0.0}
 
+    cpu = {}
+
     def loader(i, b, r):
         # Synthesise cf32 samples, run the FP64 kernel on them into the FP64
         # grids (timed per launch), then pack the resident cf16 copy.
         g = b.geom
         stg = staging.rows(0, g.L * g.S)
         synth.device_samples(b.payload, stg, r.sat, stream=stream)
+        if i == 0 and not args.no_cpu:
+            stream.sync()
+            cpu["bp"] = cpu_baseline(b.payload, stg, budget_s=10.0)
         o = out64.rows(0, g.nlines)
         c = coef.rows(0, g.nlines)
         T.fit_aperture(g, b.dem, c, stream=stream)
@@ -433,7 +478,14 @@ def stack_bench(args, synth, T):
             am.free()
         ms_pair = float(np.median(ig_ms[1:] if len(ig_ms) > 1 else ig_ms))
         px = grid.lines * grid.W
+        cpu_ig = None
+        if not args.no_cpu:
+            rows = min(256, grid.lines)
+            r0 = stk.bursts[0][0].geom.firstline + 200
+            cpu_ig = cpu_baseline_igram(g_mixed[0].rows(r0, rows).download(),
+                                        g_mixed[1].rows(r0, rows).download(), grid.lines)
         igram = {"pairs": nacq - 1, "looks": [lk, lk], "ms_per_interferogram": round(ms_pair, 3),
+                 "cpu_baseline": cpu_ig,
                  "input_gpx_per_s": round(px / ms_pair / 1e6, 2),
                  "hbm_gbps_algorithmic": round((2 * px * 8 + 2 * px * 8 / lk / lk) / ms_pair / 1e6,
                                                1)}
@@ -461,6 +513,7 @@ def stack_bench(args, synth, T):
             "extra_nonzero_pixels": sum(e["extra_nonzero"] for e in per_acq)},
         "baselines_m": [round(float(x), 2) for x in np.linalg.norm(stk.baselines, axis=1)],
         "interferograms": igram,
+        "cpu_baseline": cpu.get("bp"),
     }
     print(json.dumps(line), flush=True)
 
