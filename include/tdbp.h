@@ -388,9 +388,11 @@ int tdbp_chirp_decode(const uint8_t *header80, tdbp_chirp *c);
  *             the backprojection input (header samples zero, as
  *             sentinel/backprojectgpusub.f90:190 leaves them) in the storage
  *             of tdbp_pack_samples.
- * The transform is the reference's: 32768-point forward FFT of the
- * zero-padded echo, times conj(FFT(reference chirp / npts)), inverse FFT
- * without normalisation, in FP32. */
+ * The values are the reference's: 32768-point forward FFT of the zero-padded
+ * echo, times conj(FFT(reference chirp / npts)), inverse FFT without
+ * normalisation, in FP32.  The kept samples are a linear correlation of the
+ * echo window, so the device uses the shortest 2^a 3^b 5^c length >= nsamps
+ * (24576 for a 24400-sample window) with the 32768-point scale. */
 enum { TDBP_RC_ZERO_HEADER = 1 };
 int tdbp_range_compress(const void *d_raw, int nlines, int raw_stride,
                         const tdbp_chirp *c, void *d_out, int out_stride, int flags,

@@ -32,7 +32,7 @@ using namespace tdbp;
 
 namespace {
 
-constexpr int kFftLen = 32768;   // processsub.f90:39
+constexpr int kFftLen = 32768;   // processsub.f90:39, the largest length used
 constexpr int kChunk = 256;      // lines per batched transform
 constexpr int kHdr = 10;         // header samples (80 bytes) per line
 constexpr double kFref = 37.53472224;
@@ -69,11 +69,13 @@ double sample_frequency(int code) {
 // ------------------------------------------------------------ kernels
 
 // Reference function (processsub.f90:96-101): t in FP32 (integer over real*4),
-// phase in FP64, cos/sin rounded to FP32, divided by npts in FP32.
-__global__ void k_chirp(float2* __restrict__ ref, int npts, float samplefreq,
-                        float startfreq, float ramprate) {
+// phase in FP64, cos/sin rounded to FP32, divided by npts in FP32; times
+// gain = 32768 / n so that an n-point transform pair gives the reference's
+// 32768-point output scale (1 when n = 32768).
+__global__ void k_chirp(float2* __restrict__ ref, int n, int npts, float samplefreq,
+                        float startfreq, float ramprate, float gain) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= kFftLen) return;
+  if (j >= n) return;
   float2 v = make_float2(0.f, 0.f);
   if (j < npts) {
 #pragma clang fp contract(off)
@@ -81,27 +83,29 @@ __global__ void k_chirp(float2* __restrict__ ref, int npts, float samplefreq,
     const double t = (double)((float)j / samplefreq);
     const double phase = 2.0 * pi * (double)startfreq * t + pi * (double)ramprate * t * t;
     const float fn = (float)npts;
-    v = make_float2((float)cos(phase) / fn, (float)sin(phase) / fn);
+    v = make_float2((float)cos(phase) / fn * gain, (float)sin(phase) / fn * gain);
   }
   ref[j] = v;
 }
 
 // Zero-padded echo samples of lines [base, base + kChunk).
 __global__ void k_rc_load(const float2* __restrict__ raw, int nlines, int stride, int nsamps,
-                          int base, float2* __restrict__ buf) {
+                          int base, int n, float2* __restrict__ buf) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;   // sample
   const int i = blockIdx.y;                              // line in chunk
+  if (j >= n) return;
   const int l = base + i;
   float2 v = make_float2(0.f, 0.f);
   if (l < nlines && j < nsamps) v = raw[(size_t)l * stride + kHdr + j];
-  buf[(size_t)i * kFftLen + j] = v;
+  buf[(size_t)i * n + j] = v;
 }
 
 // Spectrum times conj(reference spectrum).
-__global__ void k_rc_mul(float2* __restrict__ buf, const float2* __restrict__ ref) {
+__global__ void k_rc_mul(float2* __restrict__ buf, const float2* __restrict__ ref, int n) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   const int i = blockIdx.y;
-  float2* p = buf + (size_t)i * kFftLen + j;
+  if (j >= n) return;
+  float2* p = buf + (size_t)i * n + j;
   const float2 a = *p, r = ref[j];
   *p = make_float2(a.x * r.x + a.y * r.y, a.y * r.x - a.x * r.y);
 }
@@ -111,13 +115,13 @@ __global__ void k_rc_mul(float2* __restrict__ buf, const float2* __restrict__ re
 constexpr int kStoreBlock = 1024;
 __global__ void __launch_bounds__(kStoreBlock)
 k_rc_store(const float2* __restrict__ raw, int nlines, int stride, int base,
-           const float2* __restrict__ buf, int nvalid, float2* __restrict__ out,
+           const float2* __restrict__ buf, int n, int nvalid, float2* __restrict__ out,
            int out_stride, int zero_header, uint32_t* __restrict__ packed,
            float* __restrict__ scale, int dtype) {
   const int i = blockIdx.x;
   const int l = base + i;
   if (l >= nlines) return;
-  const float2* row = buf + (size_t)i * kFftLen;
+  const float2* row = buf + (size_t)i * n;
   if (out) {
     float2* o = out + (size_t)l * out_stride;
     for (int k = threadIdx.x; k < kHdr + nvalid; k += kStoreBlock)
@@ -156,8 +160,9 @@ k_rc_store(const float2* __restrict__ raw, int nlines, int stride, int base,
 // Per device: the transform buffer of one batch of lines and the reference
 // spectrum of the last chirp seen.
 struct RcState {
-  float2* buf = nullptr;   // [kChunk][kFftLen]
-  float2* ref = nullptr;   // [kFftLen]
+  float2* buf = nullptr;   // [kChunk][n], n <= kFftLen
+  float2* ref = nullptr;   // [n] reference spectrum for ref_of at length ref_n
+  int ref_n = 0;
   tdbp_chirp ref_of;
   bool ref_valid = false;
 };
@@ -181,6 +186,20 @@ int state_for(int dev, RcState** out) {
 
 bool same_chirp(const tdbp_chirp& a, const tdbp_chirp& b) {
   return std::memcmp(&a, &b, sizeof a) == 0;
+}
+
+// Transform length: the smallest 2^a 3^b 5^c >= nsamps.  The kept samples
+// k < nvalid = nsamps - npts are sums over echo samples k .. k + npts - 1 <
+// nsamps only, so any length >= nsamps gives the reference's 32768-point
+// values; a shorter one moves fewer bytes (24576 for a 24400-sample window).
+int transform_length(int nsamps) {
+  for (int n = nsamps; n < kFftLen; ++n) {
+    int m = n;
+    for (int p : {2, 3, 5})
+      while (m % p == 0) m /= p;
+    if (m == 1) return n;
+  }
+  return kFftLen;
 }
 
 }  // namespace
@@ -236,30 +255,32 @@ int tdbp_range_compress(const void* d_raw, int nlines, int raw_stride, const tdb
   RcState* r = nullptr;
   if ((st = state_for(dev, &r))) return st;
   hipStream_t s = (hipStream_t)stream;
-  if (!r->ref_valid || !same_chirp(r->ref_of, *c)) {
-    hipLaunchKernelGGL(k_chirp, dim3(kFftLen / 256), dim3(256), 0, s, r->ref, c->npts,
-                       c->samplefreq, c->startfreq, c->ramprate);
+  const int n = transform_length(c->nsamps);
+  const unsigned nb = (unsigned)((n + 255) / 256);
+  if (!r->ref_valid || !same_chirp(r->ref_of, *c) || r->ref_n != n) {
+    hipLaunchKernelGGL(k_chirp, dim3(nb), dim3(256), 0, s, r->ref, n, c->npts, c->samplefreq,
+                       c->startfreq, c->ramprate, (float)((double)kFftLen / n));
     if ((st = hip_check(hipGetLastError(), "chirp launch")) ||
-        (st = fft_run(kFftLen, 1, false, r->ref, s)))
+        (st = fft_run(n, 1, false, r->ref, s)))
       return st;
     r->ref_of = *c;
+    r->ref_n = n;
     r->ref_valid = true;
   }
   const float2* raw = (const float2*)d_raw;
   for (int base = 0; base < nlines; base += kChunk) {
-    hipLaunchKernelGGL(k_rc_load, dim3(kFftLen / 256, kChunk), dim3(256), 0, s, raw, nlines,
-                       raw_stride, c->nsamps, base, r->buf);
+    hipLaunchKernelGGL(k_rc_load, dim3(nb, kChunk), dim3(256), 0, s, raw, nlines, raw_stride,
+                       c->nsamps, base, n, r->buf);
     if ((st = hip_check(hipGetLastError(), "load launch")) ||
-        (st = fft_run(kFftLen, kChunk, false, r->buf, s)))
+        (st = fft_run(n, kChunk, false, r->buf, s)))
       return st;
-    hipLaunchKernelGGL(k_rc_mul, dim3(kFftLen / 256, kChunk), dim3(256), 0, s, r->buf,
-                       r->ref);
+    hipLaunchKernelGGL(k_rc_mul, dim3(nb, kChunk), dim3(256), 0, s, r->buf, r->ref, n);
     if ((st = hip_check(hipGetLastError(), "product launch")) ||
-        (st = fft_run(kFftLen, kChunk, true, r->buf, s)))
+        (st = fft_run(n, kChunk, true, r->buf, s)))
       return st;
     const int nchunk = nlines - base < kChunk ? nlines - base : kChunk;
     hipLaunchKernelGGL(k_rc_store, dim3(nchunk), dim3(kStoreBlock), 0, s, raw, nlines,
-                       raw_stride, base, r->buf, c->nvalid, (float2*)d_out, out_stride,
+                       raw_stride, base, r->buf, n, c->nvalid, (float2*)d_out, out_stride,
                        (flags & TDBP_RC_ZERO_HEADER) ? 1 : 0, (uint32_t*)d_packed, d_scale,
                        dtype);
     if ((st = hip_check(hipGetLastError(), "store launch"))) return st;

@@ -100,3 +100,16 @@ def test_point_target_compresses_to_n():
     assert abs(out[d] / RO.NFFT - 1) < 1e-6            # N * sum |chirp|^2 / npts = N
     side = np.abs(np.delete(out, np.arange(d - 2, d + 3)))
     assert side.max() < 0.3 * RO.NFFT
+
+
+def test_shorter_transform_gives_the_reference_samples():
+    """The kept samples are a linear correlation of the echo window: any
+    transform length >= nsamps gives the 32768-point values up to the length
+    factor of the unnormalised inverse (the device uses 24576 for IW1)."""
+    c = RO.chirp_params(raw.header_bytes(raw.IW1))
+    rng = np.random.default_rng(9)
+    lines = (rng.standard_normal((2, 10 + c["nsamps"])) +
+             1j * rng.standard_normal((2, 10 + c["nsamps"]))).astype(np.complex64)
+    full = RO.compress(lines, c)[:, RO.HDR:]
+    short = RO.compress(lines, c, n=24576)[:, RO.HDR:] * (32768 / 24576)
+    np.testing.assert_allclose(short, full, rtol=0, atol=1e-9 * np.abs(full).max())
