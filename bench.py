@@ -107,6 +107,43 @@ def cpu_baseline(sc, d_data, budget_s=15.0):
                       f"OMP threads {threads}"}
 
 
+def cpu_baseline_entry(sc, data, pairs, td):
+    """The reference's whole CPU entry azimuth_compress_cpu_ (oracle/_ref,
+    built from its own sources, gcc -O2 -fopenmp) on the same burst, DEM file
+    and .geo file as the drop-in: one full call, file I/O included.  None when
+    the reference build is absent."""
+    import ctypes
+    from oracle import pyoracle
+
+    if not pyoracle.ref_available():
+        return None
+    ref = pyoracle.RefCPU()
+    g = sc.geom
+    demf, geof = os.path.join(td, "ref.dem"), os.path.join(td, "ref.geo")
+    sc.dem.astype(np.int16).tofile(demf)
+    np.zeros((sc.demlength, g.demwidth), np.complex64).tofile(geof)
+    fdd, fdo = os.open(demf, os.O_RDONLY), os.open(geof, os.O_RDWR)
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or os.cpu_count()
+    try:
+        t = time.perf_counter()
+        ref.azimuth_compress_cpu(data, sc.satloc, g.L, g.S, g.demwidth, sc.demlength, fdo, fdd,
+                                 g.deltalat, g.deltalon, g.firstlat, g.firstlon, sc.latlons,
+                                 g.timeorbit, g.xx, g.vv, g.rngstart, g.rngend, g.tmid,
+                                 g.xyz_mid, g.vel_mid, g.dmrg, g.wvl, g.A, g.angc0, g.angc1,
+                                 g.prf)
+        ctypes.CDLL(None).fflush(None)
+        dt = time.perf_counter() - t
+    finally:
+        os.close(fdd)
+        os.close(fdo)
+    return {"value": pairs / dt / 1e9, "unit": "Gpixel·pulse/s", "cores": threads,
+            "kind": "reference",
+            "sample": f"one full azimuth_compress_cpu_ call on the same burst ({g.L}x{g.S} -> "
+                      f"{g.nlines}x{g.demwidth} px, {pairs} pairs) and files in {dt:.1f} s, "
+                      f"DEM read and .geo read-modify-write included, gcc -O2 -fopenmp, "
+                      f"OMP threads {threads}"}
+
+
 def cpu_baseline_rc(lines, chirp_header, budget_s=10.0):
     """Range compression by the oracle (numpy FFT in FP64, one thread) on a
     bounded sample: the given lines, repeated until about budget_s of work.
@@ -276,13 +313,30 @@ def dropin_bench(args, synth, T):
                 log(f"drop-in {name}: {ms:.2f} ms/call ({pairs / ms / 1e6:.1f} GP/s end to "
                     f"end); async {ams:.2f} ms/call ({pairs / ams / 1e6:.1f} GP/s); "
                     f"phases {ph}")
+            # the CPU twin's entry (azimuth_compress_cpu_): the same burst with the
+            # twin's arithmetic (FP64 lerp, unscaled, integer-abs merge)
+            os.environ["TDBP_PRECISION"] = "fp64"
+            for _ in range(args.warmup):
+                call.call(fdo, fdd, entry="cpu")
+            walls = []
+            for _ in range(args.steps):
+                t0 = time.perf_counter()
+                call.call(fdo, fdd, entry="cpu")
+                walls.append((time.perf_counter() - t0) * 1e3)
+            ms = float(np.median(walls))
+            res["cpu_twin_fp64"] = {"gpps_end_to_end": round(pairs / ms / 1e6, 2),
+                                    "ms_per_call": round(ms, 3)}
+            log(f"drop-in azimuth_compress_cpu_ fp64: {ms:.2f} ms/call "
+                f"({pairs / ms / 1e6:.1f} GP/s end to end)")
         finally:
             os.environ.pop("TDBP_PRECISION", None)
             os.environ.pop("TDBP_DROPIN_ASYNC", None)
             os.close(fdd)
             os.close(fdo)
+        cpu = None if args.no_cpu else cpu_baseline_entry(sc, data, pairs, td)
     best = res["mixed_fp16"]
-    cpu = None if args.no_cpu else cpu_baseline(sc, data)
+    if cpu is None and not args.no_cpu:
+        cpu = cpu_baseline(sc, data)
     line = {"metric": METRIC + " (drop-in, end to end)", "value": best["gpps_end_to_end"],
             "unit": "Gpixel·pulse/s", "n_gpus": 1, "steps": args.steps,
             "warmup": args.warmup, "ms_per_step": best["ms_per_call"],

@@ -50,6 +50,22 @@ typedef enum {
   TDBP_CBF16 = 2        /* complex bfloat16 + FP32 scale per pulse, 4 B    */
 } tdbp_dtype;
 
+/* Which reference entry's arithmetic a burst follows.  The two reference
+ * implementations of the path differ in three places:
+ *   GPU entry (sentinel/azimuth_compress.cu:61-79): FP32 lerp weights and
+ *     products, output multiplied by float(1/(azend-azstart)); .geo pixel
+ *     replaced where fabs((double)Re) > 1e-18 (:527-531).
+ *   CPU twin (sentinel/azimuth_compress_cpu.c:68-82): lerp in FP64 rounded to
+ *     cf32, phase 4*pi/wvl*range in FP64, no scaling; .geo pixel replaced
+ *     where abs((int)Re) > 1e-18 (:386-388), which for the reference's own
+ *     build (gcc without -O, build_proc:10) means 1 <= |Re| < 2^31 (x86's
+ *     truncating conversion gives INT_MIN beyond that, and abs keeps it
+ *     negative).  A window of one line is skipped (:254-257). */
+typedef enum {
+  TDBP_SEM_GPU = 0,
+  TDBP_SEM_CPU = 1
+} tdbp_semantics;
+
 /*
  * Burst geometry: the scalars the reference driver derives per burst
  * (sentinel/backprojectgpusub.f90:193-235) and the DEM window the entry
@@ -72,6 +88,8 @@ typedef struct tdbp_geom {
   int32_t fit_last_line;  /* 0: like the reference loop, the last window
                              line gets no aperture fit and stays zero; 1:
                              fit it like the others                       */
+  int32_t semantics;      /* tdbp_semantics: output scale and lerp of the
+                             GPU entry (0) or of the CPU twin (1)         */
 } tdbp_geom;
 
 /* ---------------------------------------------------------------- drop-in */
@@ -96,7 +114,26 @@ void azimuth_compress_(void *burstdata, double *satloc, int *rawdatalines,
                        int *aperture, int *iaperture, double *angc0,
                        double *angc1, double *prf);
 
-/* Status of the last azimuth_compress_ call on this thread (the reference
+/* Replaces the CPU twin sentinel/azimuth_compress_cpu.c:178-213 (called
+ * from sentinel/backprojectcpusub.f90:243-246) with the same 33 arguments:
+ * the sum runs on the device with the CPU twin's arithmetic
+ * (TDBP_SEM_CPU: FP64 lerp rounded to cf32, no 1/(azend-azstart) scale), the
+ * .geo merge keeps its integer-abs rule (1 <= |Re| < 2^31), and a burst whose
+ * DEM window is one line is skipped with the reference's message.  Same
+ * environment knobs and status calls as azimuth_compress_. */
+void azimuth_compress_cpu_(void *burstdata, double *satloc, int *rawdatalines,
+                           int *samplesPerBurst, int *demwidth, int *demlength,
+                           int *fdout, int *fddem, double *deltalat,
+                           double *deltalon, double *firstlat, double *firstlon,
+                           double *latlons, double *timeorbit, double *xx,
+                           double *vv, int *numstatevec, double *rngstart,
+                           double *rngend, double *tstart, double *tend,
+                           double *tmid, double *xyz_mid, double *vel_mid,
+                           double *t, double *dtaz, double *dmrg, double *wvl,
+                           int *aperture, int *iaperture, double *angc0,
+                           double *angc1, double *prf);
+
+/* Status of the last azimuth_compress_ / azimuth_compress_cpu_ call on this thread (the reference
  * entry returns nothing). */
 int tdbp_dropin_status(void);
 
@@ -196,10 +233,12 @@ typedef struct tdbp_samples {
 int tdbp_pack_samples(const void *d_cf32, int L, int S, int dtype,
                       void *d_packed, float *d_scale, void *stream);
 
-/* pixelint on materialised inputs (sentinel/azimuth_compress.cu:33-82):
+/* pixelint on materialised inputs (sentinel/azimuth_compress.cu:33-82, or
+ * sentinel/azimuth_compress_cpu.c:40-86 when g->semantics is TDBP_SEM_CPU):
  * d_xyz f64 [nlines][W][3], d_azoff f64 [nlines][W], cf32 samples,
- * d_satloc f64 [L][3]; d_out cf32 [nlines][W] scaled by 1/(azend-azstart).
- * FP64 range and phase, FP32 lerp. */
+ * d_satloc f64 [L][3]; d_out cf32 [nlines][W].  FP64 range and phase; GPU
+ * entry: FP32 lerp, output scaled by 1/(azend-azstart); CPU twin: FP64 lerp
+ * rounded to cf32, unscaled. */
 int tdbp_pixelint(const tdbp_geom *g, const void *d_burst,
                   const double *d_satloc, const double *d_xyz,
                   const double *d_azoff, void *d_out, void *stream);
@@ -208,7 +247,7 @@ int tdbp_pixelint(const tdbp_geom *g, const void *d_burst,
  * sum, one launch after the fit.  d_coef: from tdbp_fit_aperture (NULL: fit
  * here).  d_out: cf32 [nlines][W]; pixels outside [firstpix, lastpix] or
  * with no valid window are written 0.  Output scaled by 1/(azend-azstart)
- * as the reference GPU kernel does. */
+ * as the reference GPU kernel does, unscaled under TDBP_SEM_CPU. */
 int tdbp_backproject(const tdbp_geom *g, const tdbp_samples *s,
                      const double *d_satloc, const int16_t *d_dem,
                      const double *d_coef, void *d_out, int precision,
@@ -251,8 +290,10 @@ int tdbp_batch_add(tdbp_batch *b, const tdbp_geom *g, const tdbp_samples *s,
 /* Merge burst `burst`'s output (its [nlines][W] rows) into rows
  * [firstline, firstline+nlines) of a device grid [grid_lines][grid_width]
  * (grid_width must equal the burst's demwidth) after every run, in burst
- * order, with the .geo rule of sentinel/azimuth_compress.cu:527-531 (pixel
- * replaced where |Re| > 1e-18).  Set before the first run.  Replaces the
+ * order, with the .geo rule of the burst's semantics (GPU entry:
+ * sentinel/azimuth_compress.cu:527-531, pixel replaced where |Re| > 1e-18;
+ * CPU twin: sentinel/azimuth_compress_cpu.c:386-388, 1 <= |Re| < 2^31).
+ * Set before the first run.  Replaces the
  * per-burst .geo read-modify-write of sentinel/azimuth_compress.cu:513-536. */
 int tdbp_batch_set_merge(tdbp_batch *b, int burst, void *d_grid, int grid_lines,
                          int grid_width, int firstline);
@@ -267,10 +308,13 @@ size_t tdbp_batch_device_bytes(const tdbp_batch *b);
 void tdbp_batch_destroy(tdbp_batch *b);
 
 /* The .geo merge on the device: burst output d_out [nlines][W] into rows
- * [firstline, firstline+nlines) of d_grid [grid_lines][W], replacing a pixel
- * where |Re(out)| > 1e-18 (sentinel/azimuth_compress.cu:527-531). */
+ * [firstline, firstline+nlines) of d_grid [grid_lines][W].  semantics
+ * TDBP_SEM_GPU replaces a pixel where |Re(out)| > 1e-18
+ * (sentinel/azimuth_compress.cu:527-531); TDBP_SEM_CPU where
+ * 1 <= |Re(out)| < 2^31 (sentinel/azimuth_compress_cpu.c:386-388 as the
+ * reference builds it). */
 int tdbp_merge_geo(void *d_grid, int W, int grid_lines, int firstline, int nlines,
-                   const void *d_out, void *stream);
+                   const void *d_out, int semantics, void *stream);
 
 /* Error sums of a cf32 image against a cf32 reference, n pixels, over the
  * pixels where the reference is nonzero: sums[0] = sum |ref|^2 dphi^2,

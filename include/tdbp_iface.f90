@@ -1,5 +1,5 @@
 ! tdbp_iface.f90 -- explicit Fortran interface of the TDBP engine's drop-in
-! entry.  The reference burst driver (sentinel/backprojectgpusub.f90:244-247)
+! entries (azimuth_compress and its CPU twin azimuth_compress_cpu).  The reference burst driver (sentinel/backprojectgpusub.f90:244-247)
 ! calls azimuth_compress with an implicit interface; compiling it together
 ! with this module (or with nothing at all: the external name is the same)
 ! binds the call to libsrg_tdbp.so.  All 33 arguments by reference, as the
@@ -26,6 +26,27 @@ module tdbp_iface
        integer(c_int) :: aperture, iaperture
        real(c_double) :: angc0, angc1, prf
      end subroutine azimuth_compress
+
+     ! The CPU twin's entry (sentinel/backprojectcpusub.f90:243-246), same
+     ! arguments; the sum runs on the device with the twin's arithmetic.
+     subroutine azimuth_compress_cpu(burstdata, satloc, rawdatalines, samplesperburst, &
+          demwidth, demlength, fdout, fddem, deltalat, deltalon, firstlat, firstlon, &
+          latlons, timeorbit, xx, vv, numstatevec, rngstart, rngend, tstart, tend, &
+          tmid, xyz_mid, vel_mid, t, dtaz, dmrg, wvl, aperture, iaperture, angc0, &
+          angc1, prf) bind(C, name="azimuth_compress_cpu_")
+       import :: c_float_complex, c_double, c_int
+       complex(c_float_complex) :: burstdata(*)
+       real(c_double) :: satloc(3, *)
+       integer(c_int) :: rawdatalines, samplesperburst, demwidth, demlength
+       integer(c_int) :: fdout, fddem
+       real(c_double) :: deltalat, deltalon, firstlat, firstlon
+       real(c_double) :: latlons(4), timeorbit(*), xx(3, *), vv(3, *)
+       integer(c_int) :: numstatevec
+       real(c_double) :: rngstart, rngend, tstart, tend, tmid
+       real(c_double) :: xyz_mid(3), vel_mid(3), t(*), dtaz, dmrg, wvl
+       integer(c_int) :: aperture, iaperture
+       real(c_double) :: angc0, angc1, prf
+     end subroutine azimuth_compress_cpu
 
      integer(c_int) function tdbp_dropin_status() bind(C, name="tdbp_dropin_status")
        import :: c_int

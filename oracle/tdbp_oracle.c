@@ -346,13 +346,19 @@ int64_t orc_backproject(const float *burst, const double *satloc,
 }
 
 /* The .geo merge of one burst window: grid pixel k takes out[k] where the
- * merge test on Re(out[k]) passes.  CPU twin: abs((int)re) > 1e-18, i.e. only
- * |re| >= 1 (sentinel/azimuth_compress_cpu.c:387); GPU entry:
- * fabs((double)re) > 1e-18 (sentinel/azimuth_compress.cu:527-531). */
+ * merge test on Re(out[k]) passes.  GPU entry: fabs((double)re) > 1e-18
+ * (sentinel/azimuth_compress.cu:527-531).  CPU twin: abs((int)re) > 1e-18
+ * (sentinel/azimuth_compress_cpu.c:387), whose conversion is undefined in C
+ * beyond the int range; the reference's own build (gcc without -O,
+ * build_proc:10) runs x86's truncating conversion, which gives INT_MIN for
+ * |re| >= 2^31 and NaN, and abs(INT_MIN) stays negative.  Written out here
+ * without the undefined conversion: 1 <= |re| < 2^31 (the live reference
+ * builds pin both ends; an optimising gcc build takes |re| >= 2^31 too). */
 void orc_merge_geo(float *grid, const float *out, long npx, int semantics) {
   for (long k = 0; k < npx; ++k) {
     float re = out[2 * k];
-    int take = semantics == ORC_CPU_SEMANTICS ? (abs((int)re) > 1.e-18)
+    float a = fabsf(re);
+    int take = semantics == ORC_CPU_SEMANTICS ? (a >= 1.f && a < 2147483648.f)
                                               : (fabs((double)re) > 1.e-18);
     if (take) {
       grid[2 * k] = out[2 * k];

@@ -112,6 +112,8 @@ int validate_geom(const tdbp_geom* g) {
                      g->firstpix, g->lastpix, g->demwidth);
   if (!(g->dmrg > 0.) || !(g->wvl > 0.) || !(g->prf > 0.))
     return set_error(TDBP_ERR_ARG, "dmrg/wvl/prf must be positive");
+  if (g->semantics != TDBP_SEM_GPU && g->semantics != TDBP_SEM_CPU)
+    return set_error(TDBP_ERR_ARG, "unknown semantics %d", g->semantics);
   // The gate rngstart <= R <= rngend - dmrg must keep sample r+1 inside the
   // row: (rngend - dmrg - rngstart)/dmrg <= S - 2 (+ rounding).
   const double rhi = (g->rngend - g->dmrg - g->rngstart) / g->dmrg;
@@ -138,6 +140,9 @@ void fill_burst(const tdbp_geom* g, BurstDev* b) {
   b->rng_hi = g->rngend - g->dmrg;
   b->inv_dmrg = 1.0 / g->dmrg;
   b->two_over_wvl = 2.0 / g->wvl;
+  b->cpu = g->semantics == TDBP_SEM_CPU;
+  b->dmrg = g->dmrg;
+  b->four_pi_over_wvl = 4. * kPi / g->wvl;  // sentinel/azimuth_compress_cpu.c:75
   b->r_hi = std::min((g->rngend - g->dmrg - g->rngstart) / g->dmrg,
                      (double)(g->samples - 2));
   b->firstlat = g->firstlat;
@@ -313,15 +318,17 @@ int tdbp_memcpy_dtod(void* d, const void* src, size_t n, void* s) {
 }
 
 int tdbp_merge_geo(void* d_grid, int W, int grid_lines, int firstline, int nlines,
-                   const void* d_out, void* stream) {
+                   const void* d_out, int semantics, void* stream) {
   if (!d_grid || !d_out || W < 1 || nlines < 1 || firstline < 0 ||
       firstline + nlines > grid_lines)
     return set_error(TDBP_ERR_ARG, "merge: rows [%d, %d) outside a %d-line grid", firstline,
                      firstline + nlines, grid_lines);
+  if (semantics != TDBP_SEM_GPU && semantics != TDBP_SEM_CPU)
+    return set_error(TDBP_ERR_ARG, "merge: unknown semantics %d", semantics);
   int st = current_device_ready();
   if (st) return st;
   return hip_check(launch_merge((float2*)d_grid, (const float2*)d_out, (size_t)nlines * W,
-                                (size_t)firstline * W, (hipStream_t)stream),
+                                (size_t)firstline * W, semantics, (hipStream_t)stream),
                    "merge launch");
 }
 

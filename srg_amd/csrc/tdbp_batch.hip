@@ -73,7 +73,8 @@ int enqueue(tdbp_batch* b, hipStream_t s) {
     if (!m.grid) continue;
     const BurstDev& bd = b->bursts[i];
     st = hip_check(launch_merge(m.grid, bd.out, (size_t)bd.nlines * bd.W,
-                                (size_t)m.firstline * bd.W, s),
+                                (size_t)m.firstline * bd.W,
+                                bd.cpu ? TDBP_SEM_CPU : TDBP_SEM_GPU, s),
                    "merge launch");
     if (st) return st;
   }

@@ -1,6 +1,11 @@
 // tdbp_dropin.hip -- azimuth_compress_, the symbol the reference Fortran
 // burst driver calls (sentinel/backprojectgpusub.f90:244-247), with the
-// semantics of sentinel/azimuth_compress.cu:181-562:
+// semantics of sentinel/azimuth_compress.cu:181-562, and its CPU twin
+// azimuth_compress_cpu_ (sentinel/backprojectcpusub.f90:243-246,
+// sentinel/azimuth_compress_cpu.c:178-403), which runs on the device too but
+// keeps the twin's arithmetic: FP64 lerp rounded to cf32, no 1/N scale, the
+// integer-abs .geo rule (1 <= |Re| < 2^31) and the skip of a one-line
+// window (:254-257).  The GPU entry's steps:
 //   * DEM lines and pixels of the burst from latlons (:295-301, :378-379),
 //     the "Burst line limits" line on stdout (:302);
 //   * the DEM slice read from fddem at firstline*W*2 (:400-403);
@@ -64,6 +69,7 @@ struct GeoJob {
   off_t off;
   size_t npx;
   int firstline;
+  int semantics;   // merge rule: TDBP_SEM_GPU or TDBP_SEM_CPU
 };
 
 std::mutex g_mu;                 // one call at a time, as the reference driver makes them
@@ -164,8 +170,8 @@ bool write_full(int fd, const void* buf, size_t n, off_t off) {
 // The .geo read-merge-write of one burst whose sum is enqueued on the slot's
 // stream: rows read into pinned memory (overlapping the sum), merged on the
 // device, written back.  times (optional): read, merge round trip, write ms.
-int geo_rmw(Slot& sl, int fdout, off_t off, size_t npx, int firstline, double* times,
-            char* msg, size_t msglen) {
+int geo_rmw(Slot& sl, int fdout, off_t off, size_t npx, int firstline, int semantics,
+            double* times, char* msg, size_t msglen) {
   float2* hgeo = (float2*)sl.h[kHostGeo];
   const double t2 = now_ms();
   const ssize_t gg = read_full(fdout, hgeo, npx * 8, off);
@@ -179,7 +185,7 @@ int geo_rmw(Slot& sl, int fdout, off_t off, size_t npx, int firstline, double* t
   const double t3 = now_ms();
   (void)hipEventRecord(sl.ev[3], sl.s);
   if (hipMemcpyAsync(sl.p[kGeo], hgeo, npx * 8, hipMemcpyHostToDevice, sl.s) != hipSuccess ||
-      launch_merge((float2*)sl.p[kGeo], (const float2*)sl.p[kOut], npx, 0, sl.s) !=
+      launch_merge((float2*)sl.p[kGeo], (const float2*)sl.p[kOut], npx, 0, semantics, sl.s) !=
           hipSuccess ||
       hipMemcpyAsync(hgeo, sl.p[kGeo], npx * 8, hipMemcpyDeviceToHost, sl.s) != hipSuccess) {
     (void)hipStreamSynchronize(sl.s);
@@ -217,8 +223,8 @@ void worker_main() {
       j = g_q.front();
     }
     char msg[256] = "";
-    const int st = geo_rmw(g_slot[j.slot], j.fdout, j.off, j.npx, j.firstline, nullptr, msg,
-                           sizeof msg);
+    const int st = geo_rmw(g_slot[j.slot], j.fdout, j.off, j.npx, j.firstline, j.semantics,
+                           nullptr, msg, sizeof msg);
     if (st != TDBP_OK)
       std::fprintf(stderr, "azimuth_compress: burst .geo not updated: %s\n", msg);
     {
@@ -273,7 +279,7 @@ int run(const float* burst, const double* satloc, int L, int S, int W,
         const double* timeorbit, const double* xx, const double* vv, int nsv,
         double rngstart, double rngend, double tmid, const double* xyz_mid,
         const double* vel_mid, double dmrg, double wvl, int aperture,
-        double angc0, double angc1, double prf) {
+        double angc0, double angc1, double prf, int semantics) {
   int st = ensure_device(0);  // the first visible device, as cudaSetDevice(0)
   if (st) return st;
   int firstline, lastline, firstpix, lastpix;
@@ -282,6 +288,14 @@ int run(const float* burst, const double* satloc, int L, int S, int W,
   const int nlines = lastline - firstline + 1;
   std::printf("Burst line limits, size (first,last,nlines) %d %d %d\n",
               firstline, lastline, nlines);
+  if (semantics == TDBP_SEM_CPU && nlines == 1) {
+    // sentinel/azimuth_compress_cpu.c:254-257: nothing read or written.
+    std::printf("Burst not in DEM, skipped.\n");
+    std::fflush(stdout);
+    std::lock_guard<std::mutex> lk(g_mu);
+    for (double& t : g_timing) t = 0.0;
+    return TDBP_OK;
+  }
   std::fflush(stdout);
 
   tdbp_geom g;
@@ -314,6 +328,7 @@ int run(const float* burst, const double* satloc, int L, int S, int W,
   g.firstlon = firstlon;
   g.deltalat = deltalat;
   g.deltalon = deltalon;
+  g.semantics = semantics;
   const char* fl = std::getenv("TDBP_FIT_LAST_LINE");
   g.fit_last_line = (fl && fl[0] == '1') ? 1 : 0;
   if ((st = validate_geom(&g))) return st;
@@ -398,7 +413,7 @@ int run(const float* burst, const double* satloc, int L, int S, int W,
     {
       std::lock_guard<std::mutex> ql(g_q_mu);
       sl.busy = true;
-      g_q.push_back(GeoJob{si, fdout, off, npx, firstline});
+      g_q.push_back(GeoJob{si, fdout, off, npx, firstline, semantics});
     }
     g_q_cv.notify_all();
     g_next ^= 1;
@@ -410,7 +425,7 @@ int run(const float* burst, const double* satloc, int L, int S, int W,
 
   double rt[3] = {0.0, 0.0, 0.0};
   char msg[256] = "";
-  if ((st = geo_rmw(sl, fdout, off, npx, firstline, rt, msg, sizeof msg)))
+  if ((st = geo_rmw(sl, fdout, off, npx, firstline, semantics, rt, msg, sizeof msg)))
     return set_error(st, "%s", msg);
   float up = 0.f, comp = 0.f;
   (void)hipEventElapsedTime(&up, ev[0], ev[1]);
@@ -449,9 +464,35 @@ void azimuth_compress_(void* burstdata, double* satloc, int* rawdatalines,
                  *demwidth, *demlength, *fdout, *fddem, *deltalat, *deltalon,
                  *firstlat, *firstlon, latlons, timeorbit, xx, vv, *numstatevec,
                  *rngstart, *rngend, *tmid, xyz_mid, vel_mid, *dmrg, *wvl,
-                 *aperture, *angc0, *angc1, *prf);
+                 *aperture, *angc0, *angc1, *prf, TDBP_SEM_GPU);
   if (g_status != TDBP_OK)
     std::fprintf(stderr, "azimuth_compress: burst skipped: %s\n",
+                 tdbp_last_error());
+}
+
+void azimuth_compress_cpu_(void* burstdata, double* satloc, int* rawdatalines,
+                           int* samplesPerBurst, int* demwidth, int* demlength,
+                           int* fdout, int* fddem, double* deltalat,
+                           double* deltalon, double* firstlat, double* firstlon,
+                           double* latlons, double* timeorbit, double* xx,
+                           double* vv, int* numstatevec, double* rngstart,
+                           double* rngend, double* tstart, double* tend,
+                           double* tmid, double* xyz_mid, double* vel_mid,
+                           double* t, double* dtaz, double* dmrg, double* wvl,
+                           int* aperture, int* iaperture, double* angc0,
+                           double* angc1, double* prf) {
+  (void)tstart;
+  (void)tend;
+  (void)t;
+  (void)dtaz;
+  (void)iaperture;
+  g_status = run((const float*)burstdata, satloc, *rawdatalines, *samplesPerBurst,
+                 *demwidth, *demlength, *fdout, *fddem, *deltalat, *deltalon,
+                 *firstlat, *firstlon, latlons, timeorbit, xx, vv, *numstatevec,
+                 *rngstart, *rngend, *tmid, xyz_mid, vel_mid, *dmrg, *wvl,
+                 *aperture, *angc0, *angc1, *prf, TDBP_SEM_CPU);
+  if (g_status != TDBP_OK)
+    std::fprintf(stderr, "azimuth_compress_cpu: burst skipped: %s\n",
                  tdbp_last_error());
 }
 

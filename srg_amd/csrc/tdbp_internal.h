@@ -31,7 +31,9 @@ struct BurstDev {
   int firstline, nlines, nfit, firstpix, lastpix;
   int dtype, nsv;
   int tiles_r, tiles_c;    // tile grid of this burst for the chosen variant
+  int cpu;                 // 1: CPU-twin arithmetic (TDBP_SEM_CPU)
   double rngstart, rng_hi, inv_dmrg, two_over_wvl, r_hi;
+  double dmrg, four_pi_over_wvl;  // CPU twin: (R - rngstart)/dmrg, 4 pi/wvl * R
   double firstlat, firstlon, deltalat, deltalon;
   double tmid, wvl, angc0, angc1, prf;
   double xyz_mid[3], vel_mid[3];
@@ -73,8 +75,9 @@ hipError_t launch_copy(void* d, const void* src, size_t bytes, hipStream_t s);
 int compare_blocks(size_t n);
 hipError_t launch_compare(const float2* g, const float2* o, size_t n, double* part,
                           int blocks, hipStream_t s);
+// semantics: TDBP_SEM_GPU or TDBP_SEM_CPU merge rule.
 hipError_t launch_merge(float2* grid, const float2* out, size_t n, size_t grid_off,
-                        hipStream_t s);
+                        int semantics, hipStream_t s);
 hipError_t launch_pack(const float2* in, int L, int S, int dtype, void* out,
                        float* scale, hipStream_t s);
 hipError_t launch_synth_clutter(float2* d, int L, int S, uint64_t seed,

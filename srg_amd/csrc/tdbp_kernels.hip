@@ -109,14 +109,19 @@ __device__ __forceinline__ void pixel_ecef(const BurstDev& b, int row, int col,
 }
 
 // Reference GPU output: cf32(acc) * float(1/(azend-azstart))
-// (sentinel/azimuth_compress.cu:78-79).
+// (sentinel/azimuth_compress.cu:78-79); the CPU twin stores cf32(acc)
+// unscaled (sentinel/azimuth_compress_cpu.c:82).
 __device__ __forceinline__ void store_out(const BurstDev& b, const PixelCtx& p,
                                           float re, float im) {
   if (!p.inwin) return;
   float2 o = make_float2(0.f, 0.f);
   if (p.computed) {
-    const float s = (float)(1. / (p.aze - p.azs));
-    o = make_float2(re * s, im * s);
+    if (b.cpu) {
+      o = make_float2(re, im);
+    } else {
+      const float s = (float)(1. / (p.aze - p.azs));
+      o = make_float2(re * s, im * s);
+    }
   }
   b.out[(size_t)p.row * b.W + p.col] = o;
 }
@@ -151,7 +156,47 @@ __device__ __forceinline__ void plain_pair(const float2* __restrict__ row,
   }
 }
 
-template <int TR, bool STACK>
+// The CPU twin's pair (sentinel/azimuth_compress_cpu.c:64-79): range index
+// by division, lerp in FP64 rounded to cf32, phase 4*pi/wvl*range in FP64
+// with full-range sin/cos, FP64 complex product; contraction off, so each
+// step rounds as the reference's x86 build (no FMA) rounds it.
+__device__ __forceinline__ void cpu_pair(const float2* __restrict__ row,
+                                         const double* x, double sx, double sy,
+                                         double sz, const BurstDev& b,
+                                         double& ar, double& ai) {
+#pragma clang fp contract(off)
+  const double dx = x[0] - sx, dy = x[1] - sy, dz = x[2] - sz;
+  const double R = sqrt(dx * dx + dy * dy + dz * dz);
+  if (R >= b.rngstart && R <= b.rng_hi) {
+    const double r = (R - b.rngstart) / b.dmrg;
+    int i = (int)floor(r);
+    const double fr = r - i;
+    i = i > b.S - 2 ? b.S - 2 : i;
+    const float2 b0 = row[i], b1 = row[i + 1];
+    const double w0 = 1 - fr;
+    const float cr = (float)((double)b0.x * w0 + (double)b1.x * fr);
+    const float ci = (float)((double)b0.y * w0 + (double)b1.y * fr);
+    const double ph = b.four_pi_over_wvl * R;
+    const double pc = cos(ph), ps = sin(ph);
+    const double tr = (double)cr * pc - (double)ci * ps;
+    const double ti = (double)cr * ps + (double)ci * pc;
+    ar = ar + tr;
+    ai = ai + ti;
+  }
+}
+
+template <bool CPU>
+__device__ __forceinline__ void fp64_pair(const float2* __restrict__ row,
+                                          const double* x, double sx, double sy,
+                                          double sz, const BurstDev& b, double& ar,
+                                          double& ai) {
+  if constexpr (CPU)
+    cpu_pair(row, x, sx, sy, sz, b, ar, ai);
+  else
+    plain_pair(row, x, sx, sy, sz, b, ar, ai);
+}
+
+template <int TR, bool STACK, bool CPU = false>
 __device__ __forceinline__ void plain_tile(const BurstDev& b, int tile) {
   PixelCtx p = pixel_of<TR, STACK>(b, tile);
   double x[3] = {0., 0., 0.};
@@ -166,7 +211,7 @@ __device__ __forceinline__ void plain_tile(const BurstDev& b, int tile) {
   for (int q = lo; q < hi; ++q) {
     const double sx = sat[3 * q], sy = sat[3 * q + 1], sz = sat[3 * q + 2];
     if ((unsigned)(q - p.azs) < n)
-      plain_pair(base + (size_t)q * b.S, x, sx, sy, sz, b, ar, ai);
+      fp64_pair<CPU>(base + (size_t)q * b.S, x, sx, sy, sz, b, ar, ai);
   }
   store_out(b, p, (float)ar, (float)ai);
 }
@@ -462,10 +507,10 @@ __device__ __forceinline__ void mixed_tile(const BurstDev& b, int tile,
 
 // ----------------------------------------------------------- kernels
 
-template <int TR, bool STACK>
+template <int TR, bool STACK, bool CPU>
 __global__ __launch_bounds__(kBlock) void k_plain(BurstDev b) {
   const int nb = b.tiles_r * b.tiles_c;
-  plain_tile<TR, STACK>(b, xcd_tile(blockIdx.x, nb));
+  plain_tile<TR, STACK, CPU>(b, xcd_tile(blockIdx.x, nb));
 }
 
 // Occupancy target of the mixed kernels: 8 waves per SIMD, the hardware
@@ -511,8 +556,10 @@ __device__ __forceinline__ void batch_body(const BurstDev* __restrict__ bursts,
     const BurstDev& bd = bursts[w.x];
     if constexpr (PREC == TDBP_MIXED) {
       mixed_tile<TR, STACK, DT>(bd, w.y, rec, sh);
+    } else if (bd.cpu) {
+      plain_tile<TR, STACK, true>(bd, w.y);
     } else {
-      plain_tile<TR, STACK>(bd, w.y);
+      plain_tile<TR, STACK, false>(bd, w.y);
     }
     __syncthreads();
   }
@@ -651,12 +698,19 @@ __global__ void k_pixelint(BurstDev b, const double* __restrict__ xyz,
     pulse_window(az, b.A, b.L, &azs, &aze);
     const double x[3] = {xyz[3 * k], xyz[3 * k + 1], xyz[3 * k + 2]};
     const float2* base = (const float2*)b.data;
-    for (int q = azs; q < aze; ++q)
-      plain_pair(base + (size_t)q * b.S, x, b.satloc[3 * q],
-                 b.satloc[3 * q + 1], b.satloc[3 * q + 2], b, ar, ai);
+    for (int q = azs; q < aze; ++q) {
+      if (b.cpu)
+        cpu_pair(base + (size_t)q * b.S, x, b.satloc[3 * q], b.satloc[3 * q + 1],
+                 b.satloc[3 * q + 2], b, ar, ai);
+      else
+        plain_pair(base + (size_t)q * b.S, x, b.satloc[3 * q], b.satloc[3 * q + 1],
+                   b.satloc[3 * q + 2], b, ar, ai);
+    }
   }
   float2 o = make_float2(0.f, 0.f);
-  if (az > 0.) {
+  if (b.cpu) {
+    o = make_float2((float)ar, (float)ai);
+  } else if (az > 0.) {
     const float s = (float)(1. / (aze - azs));
     o = make_float2((float)ar * s, (float)ai * s);
   }
@@ -673,14 +727,21 @@ __global__ void k_zero16(uint4* p, size_t n16, uint8_t* tail, int ntail) {
   if (blockIdx.x == 0 && (int)threadIdx.x < ntail) tail[threadIdx.x] = 0;
 }
 
-// .geo merge of one burst output into a grid (sentinel/azimuth_compress.cu:527-531):
-// a pixel is replaced where |Re(out)| > 1e-18 (the comparison in double).
+// .geo merge of one burst output into a grid.  GPU entry
+// (sentinel/azimuth_compress.cu:527-531): a pixel is replaced where
+// |Re(out)| > 1e-18, compared in double.  CPU twin
+// (sentinel/azimuth_compress_cpu.c:386-388): abs((int)Re) > 1e-18 as the
+// reference's unoptimised x86 build evaluates it -- the truncating conversion
+// is nonzero for |Re| >= 1 and gives INT_MIN (whose abs stays negative) for
+// |Re| >= 2^31 and NaN -- so 1 <= |Re| < 2^31.
 __global__ void k_merge(float2* __restrict__ grid, const float2* __restrict__ out,
-                        size_t n, size_t grid_off) {
+                        size_t n, size_t grid_off, int cpu) {
   const size_t stride = (size_t)gridDim.x * blockDim.x;
   for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
     const float2 v = out[i];
-    if (fabs((double)v.x) > 1.e-18) grid[grid_off + i] = v;
+    const float a = fabsf(v.x);
+    const bool take = cpu ? (a >= 1.f && a < 2147483648.f) : (fabs((double)v.x) > 1.e-18);
+    if (take) grid[grid_off + i] = v;
   }
 }
 
@@ -887,7 +948,10 @@ inline unsigned blocks_for(size_t n, int bs) {
 template <int TR, bool STACK>
 hipError_t plain_launch(const BurstDev& b, hipStream_t s) {
   const int nb = b.tiles_r * b.tiles_c;
-  hipLaunchKernelGGL((k_plain<TR, STACK>), dim3(nb), dim3(kBlock), 0, s, b);
+  if (b.cpu)
+    hipLaunchKernelGGL((k_plain<TR, STACK, true>), dim3(nb), dim3(kBlock), 0, s, b);
+  else
+    hipLaunchKernelGGL((k_plain<TR, STACK, false>), dim3(nb), dim3(kBlock), 0, s, b);
   return hipGetLastError();
 }
 
@@ -1063,11 +1127,12 @@ hipError_t launch_zero(void* p, size_t bytes, hipStream_t s) {
 }
 
 hipError_t launch_merge(float2* grid, const float2* out, size_t n, size_t grid_off,
-                        hipStream_t s) {
+                        int semantics, hipStream_t s) {
   if (n == 0) return hipSuccess;
   size_t g = blocks_for(n, kBlock);
   g = g < 16384 ? g : 16384;
-  hipLaunchKernelGGL(k_merge, dim3((unsigned)g), dim3(kBlock), 0, s, grid, out, n, grid_off);
+  hipLaunchKernelGGL(k_merge, dim3((unsigned)g), dim3(kBlock), 0, s, grid, out, n, grid_off,
+                     semantics == TDBP_SEM_CPU ? 1 : 0);
   return hipGetLastError();
 }
 

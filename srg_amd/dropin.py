@@ -1,5 +1,6 @@
 """The drop-in entry azimuth_compress_ called the way the reference's Fortran
-burst driver calls it (sentinel/backprojectgpusub.f90:244-247): 33 arguments,
+burst driver calls it (sentinel/backprojectgpusub.f90:244-247), or its CPU
+twin azimuth_compress_cpu_ (sentinel/backprojectcpusub.f90:243-246): 33 arguments,
 all by reference, fixed-size Fortran arrays for satloc (3 x 100000), the
 state vectors (1000 and 3 x 1000) and the pulse times (100000), the DEM and
 the .geo grid as open POSIX file descriptors.
@@ -72,14 +73,17 @@ class BurstCall:
                    vel_mid=np.ascontiguousarray(g.vel_mid, np.float64), dmrg=g.dmrg,
                    wvl=g.wvl, aperture=g.A, angc0=g.angc0, angc1=g.angc1, prf=g.prf)
 
-    def call(self, fdout: int, fddem: int):
-        """azimuth_compress_ on this burst; raises TdbpError on a recorded
-        failure (the .geo is then unchanged)."""
+    def call(self, fdout: int, fddem: int, entry: str = "gpu"):
+        """azimuth_compress_ (entry "gpu") or azimuth_compress_cpu_ (entry
+        "cpu") on this burst; raises TdbpError on a recorded failure (the .geo
+        is then unchanged)."""
         i = lambda v: C.byref(C.c_int(int(v)))  # noqa: E731
         d = lambda v: C.byref(C.c_double(float(v)))  # noqa: E731
         dp = lambda a: a.ctypes.data_as(C.POINTER(C.c_double))  # noqa: E731
         tp = np.zeros(100000)
-        tdbp.lib().azimuth_compress_(
+        fn = {"gpu": tdbp.lib().azimuth_compress_,
+              "cpu": tdbp.lib().azimuth_compress_cpu_}[entry]
+        fn(
             self.burst.ctypes.data, dp(self.satloc), i(self.L), i(self.S), i(self.W),
             i(self.demlength), i(fdout), i(fddem), d(self.deltalat), d(self.deltalon),
             d(self.firstlat), d(self.firstlon), dp(self.latlons), dp(self.timeorbit),

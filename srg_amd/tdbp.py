@@ -23,6 +23,9 @@ OK, ERR_ARG, ERR_HIP, ERR_NODEV, ERR_IO, ERR_NOMEM, ERR_LIMIT = range(7)
 PLAIN_FP64, MIXED = 0, 1
 # tdbp_dtype
 CF32, CF16, CBF16 = 0, 1, 2
+# Arithmetic of the reference GPU entry (FP32 lerp, 1/N output scale, |Re| >
+# 1e-18 merge) or of its CPU twin (FP64 lerp, unscaled, 1 <= |Re| < 2^31).
+SEM_GPU, SEM_CPU = 0, 1
 DTYPE_NAMES = {CF32: "cf32", CF16: "cf16", CBF16: "cbf16"}
 
 
@@ -63,6 +66,7 @@ class Geom(C.Structure):
         ("deltalat", C.c_double),
         ("deltalon", C.c_double),
         ("fit_last_line", C.c_int32),
+        ("semantics", C.c_int32),
     ]
 
 
@@ -100,6 +104,11 @@ _VP = C.c_void_p
 _DP = C.POINTER(C.c_double)
 _IP = C.POINTER(C.c_int)
 _GP = C.POINTER(Geom)
+# The 33 by-reference arguments of azimuth_compress_ / azimuth_compress_cpu_
+# (sentinel/azimuth_compress.cu:181-214).
+_DROPIN_ARGS = [_VP, _DP, _IP, _IP, _IP, _IP, _IP, _IP, _DP, _DP, _DP, _DP, _DP, _DP, _DP,
+                _DP, _IP, _DP, _DP, _DP, _DP, _DP, _DP, _DP, _DP, _DP, _DP, _DP, _IP, _IP,
+                _DP, _DP, _DP]
 _SP = C.POINTER(Samples)
 
 
@@ -169,7 +178,8 @@ def _sig(lib):
         "tdbp_batch_set_merge": (C.c_int, [_VP, C.c_int, _VP, C.c_int, C.c_int, C.c_int]),
         "tdbp_dropin_timing": (C.c_int, [C.POINTER(C.c_double), C.c_int]),
         "tdbp_compare": (C.c_int, [_VP, _VP, C.c_size_t, C.POINTER(C.c_double), _VP]),
-        "tdbp_merge_geo": (C.c_int, [_VP, C.c_int, C.c_int, C.c_int, C.c_int, _VP, _VP]),
+        "tdbp_merge_geo": (C.c_int, [_VP, C.c_int, C.c_int, C.c_int, C.c_int, _VP, C.c_int,
+                                     _VP]),
         "tdbp_batch_pairs": (C.c_int64, [_VP]),
         "tdbp_batch_variant": (C.c_int, [_VP]),
         "tdbp_batch_device_bytes": (C.c_size_t, [_VP]),
@@ -178,10 +188,8 @@ def _sig(lib):
                                          C.POINTER(C.c_float), C.c_int, _VP]),
         "tdbp_synth_targets": (C.c_int, [_VP, _GP, _VP, _DP, C.POINTER(C.c_float),
                                          C.c_int, C.c_float, C.c_int, _VP]),
-        "azimuth_compress_": (None, [_VP, _DP, _IP, _IP, _IP, _IP, _IP, _IP, _DP, _DP,
-                                     _DP, _DP, _DP, _DP, _DP, _DP, _IP, _DP, _DP, _DP,
-                                     _DP, _DP, _DP, _DP, _DP, _DP, _DP, _DP, _IP, _IP,
-                                     _DP, _DP, _DP]),
+        "azimuth_compress_": (None, _DROPIN_ARGS),
+        "azimuth_compress_cpu_": (None, _DROPIN_ARGS),
     }
     for name, (res, args) in sigs.items():
         f = getattr(lib, name)
@@ -207,7 +215,7 @@ def exported_symbols():
         "tdbp_batch_run", "tdbp_batch_set_merge", "tdbp_merge_geo", "tdbp_compare", "tdbp_dropin_timing", "tdbp_batch_pairs",
         "tdbp_batch_variant",
         "tdbp_batch_device_bytes", "tdbp_batch_destroy", "tdbp_synth_clutter",
-        "tdbp_synth_targets", "azimuth_compress_",
+        "tdbp_synth_targets", "azimuth_compress_", "azimuth_compress_cpu_",
     ]
 
 
@@ -378,6 +386,7 @@ class BurstGeometry:
     deltalat: float
     deltalon: float
     fit_last_line: int = 0
+    semantics: int = 0
     _keep: list = field(default_factory=list, repr=False)
 
     def to_c(self) -> Geom:
@@ -399,6 +408,7 @@ class BurstGeometry:
         g.firstlat, g.firstlon = self.firstlat, self.firstlon
         g.deltalat, g.deltalon = self.deltalat, self.deltalon
         g.fit_last_line = self.fit_last_line
+        g.semantics = self.semantics
         return g
 
     @property
@@ -718,11 +728,13 @@ class Batch:
             pass
 
 
-def merge_geo(d_grid: DeviceArray, firstline: int, d_out: DeviceArray, stream=None):
-    """Merge a burst output [nlines][W] into grid rows [firstline, ...)."""
+def merge_geo(d_grid: DeviceArray, firstline: int, d_out: DeviceArray, stream=None,
+              semantics: int = SEM_GPU):
+    """Merge a burst output [nlines][W] into grid rows [firstline, ...) with
+    the .geo rule of the GPU entry or of the CPU twin."""
     nl, W = d_out.shape
     check(lib().tdbp_merge_geo(d_grid.ptr, W, d_grid.shape[0], firstline, nl, d_out.ptr,
-                               _s(stream)))
+                               semantics, _s(stream)))
 
 
 def compare(d_img: DeviceArray, d_ref: DeviceArray, stream=None) -> dict:

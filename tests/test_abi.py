@@ -23,7 +23,7 @@ def header_functions():
     names = re.findall(r"\b([a-z_][a-z0-9_]*)\s*\(", txt)
     skip = {"if", "sizeof", "defined"}
     return sorted({n for n in names if n not in skip and (n.startswith("tdbp_") or
-                                                          n == "azimuth_compress_")})
+                                                          n in ("azimuth_compress_", "azimuth_compress_cpu_"))})
 
 
 def test_library_exports_every_declared_symbol():
@@ -110,6 +110,9 @@ program call_tdbp
      call azimuth_compress(burst, satloc, n, n, n, n, n, n, 0d0, 0d0, 0d0, 0d0, latlons, &
           to, xx, vv, n, 0d0, 0d0, 0d0, 0d0, 0d0, xm, vm, t, 0d0, 0d0, 0d0, n, n, &
           0d0, 0d0, 0d0)
+     call azimuth_compress_cpu(burst, satloc, n, n, n, n, n, n, 0d0, 0d0, 0d0, 0d0, &
+          latlons, to, xx, vv, n, 0d0, 0d0, 0d0, 0d0, 0d0, xm, vm, t, 0d0, 0d0, 0d0, n, &
+          n, 0d0, 0d0, 0d0)
   end if
   print *, tdbp_dropin_status()
 end program call_tdbp

@@ -220,6 +220,59 @@ def test_oracle_matches_live_reference(oracle, refcpu):
             == refcpu.orbitrangetime(p, g.timeorbit, g.xx, g.vv, g.tmid, g.xyz_mid, g.vel_mid)
 
 
+
+def cpu_entry_geo(run, d, g, burst, fill):
+    """.geo after one CPU-entry call `run(args)` on tiny.npz's DEM and a .geo
+    of `fill`; stdio is flushed before the fds close (the reference's FILE* is
+    never flushed, sentinel/azimuth_compress_cpu.c:379)."""
+    import ctypes
+
+    full = d["dem_full"]
+    with tempfile.TemporaryDirectory() as td:
+        demf, geof = os.path.join(td, "dem"), os.path.join(td, "geo")
+        full.tofile(demf)
+        np.full((full.shape[0], g.demwidth), fill, np.complex64).tofile(geof)
+        fdd, fdo = os.open(demf, os.O_RDWR), os.open(geof, os.O_RDWR)
+        try:
+            run((burst, d["satloc"], g.L, g.S, g.demwidth, full.shape[0], fdo, fdd,
+                 g.deltalat, g.deltalon, float(d["firstlat_full"]), g.firstlon, d["latlons"],
+                 g.timeorbit, g.xx, g.vv, g.rngstart, g.rngend, g.tmid, g.xyz_mid, g.vel_mid,
+                 g.dmrg, g.wvl, g.A, g.angc0, g.angc1, g.prf))
+            ctypes.CDLL(None).fflush(None)
+        finally:
+            os.close(fdd)
+            os.close(fdo)
+        return np.fromfile(geof, np.complex64).reshape(full.shape[0], g.demwidth)
+
+
+@pytest.mark.skipif(not (pyoracle.ref_available(pyoracle.REF_O0_SO) and
+                         pyoracle.ref_available(pyoracle.REF_SO)),
+                    reason="reference CPU builds (oracle/_ref) not present")
+def test_cpu_merge_rule_beyond_int_range_matches_reference_build(oracle):
+    """abs((int)Re) > 1e-18 (sentinel/azimuth_compress_cpu.c:387) for sums past
+    2^31: the reference as its build script compiles it (no -O) leaves those
+    pixels, as the oracle's 1 <= |Re| < 2^31 does; an -O2 build takes them,
+    so the two builds differ exactly at |Re| >= 2^31."""
+    d = load("tiny.npz")
+    g = G(d)
+    burst = (d["burst"].astype(np.complex128) * 1e9).astype(np.complex64)
+    fill = np.complex64(7 + 7j)
+    o0 = pyoracle.RefCPU(pyoracle.REF_O0_SO)
+    o2 = pyoracle.RefCPU(pyoracle.REF_SO)
+    a = cpu_entry_geo(lambda x: o0.azimuth_compress_cpu(*x), d, g, burst, fill)
+    b = cpu_entry_geo(lambda x: oracle.azimuth_compress(*x, semantics=pyoracle.CPU_SEMANTICS),
+                      d, g, burst, fill)
+    c = cpu_entry_geo(lambda x: o2.azimuth_compress_cpu(*x), d, g, burst, fill)
+    last = d["dem_full"].shape[0] - 2   # the reference's last window line is undefined
+    np.testing.assert_array_equal(a[:last].view(np.uint32), b[:last].view(np.uint32))
+    diff = a[:last] != c[:last]
+    assert diff.sum() > 0
+    assert np.all(np.abs(c[:last][diff].real) >= 2.0 ** 31)
+    assert np.all(a[:last][diff] == fill)
+    kept = a[:last] != fill
+    assert kept.sum() > 0 and np.all(np.abs(a[:last][kept].real) < 2.0 ** 31)
+
+
 # ------------------------------------------------------------ invariants
 
 

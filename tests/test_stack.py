@@ -35,10 +35,14 @@ def test_merge_rule_gpu_entry(oracle):
 
 def test_merge_rule_cpu_twin(oracle):
     # abs((int)re) > 1e-18 (sentinel/azimuth_compress_cpu.c:387): C's integer
-    # abs of the truncated real part, so only |re| >= 1 replaces the pixel.
+    # abs of the truncated real part, so only |re| >= 1 replaces the pixel;
+    # past the int range (and for NaN) the reference's build converts to
+    # INT_MIN, whose abs stays negative: |re| >= 2^31 is left.
     vals = [0.5 + 1j, 1e-19 + 2j, -3 + 4j, 0 + 5j, -0.9 + 0j, 2e-18 + 0j, 1.0 + 0j,
-            -1.5 - 1j]
-    take = [False, False, True, False, False, False, True, True]
+            -1.5 - 1j, 2147483520.0 + 0j, -2147483520.0 + 1j, 2147483648.0 + 0j,
+            -2147483648.0 + 0j, 3e9 + 1j, complex(math.nan, 1)]
+    take = [False, False, True, False, False, False, True, True, True, True, False, False,
+            False, False]
     g = merged(oracle, vals, pyoracle.CPU_SEMANTICS)
     want = np.where(take, np.array(vals, np.complex64), np.complex64(7 + 7j))
     assert np.array_equal(g.view(np.uint32), want.view(np.uint32))
