@@ -25,6 +25,8 @@
 #include <hip/hip_fp16.h>
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "tdbp_geom.h"
 #include "tdbp_internal.h"
 #include "tdbp_pack.h"
@@ -280,13 +282,45 @@ __device__ __forceinline__ int elem_bytes() {
   return DT == TDBP_CF32 ? 8 : 4;
 }
 
-// Per-lane constants of the mixed pulse loop.
+// The two sample words of a pair as loaded (cf32: 4 floats; cf16/bf16: 2
+// packed words), and the lerp of lerp_pair on them.
+template <int DT>
+struct RawPair {
+  typedef typename std::conditional<DT == TDBP_CF32, F32x4, U32x2>::type type;
+};
+template <int DT>
+__device__ __forceinline__ float2 lerp_raw(const typename RawPair<DT>::type& v, float fr,
+                                           float sc) {
+  if constexpr (DT == TDBP_CF32) {
+    return make_float2(fmaf(fr, v.z - v.x, v.x), fmaf(fr, v.w - v.y, v.y));
+  } else if constexpr (DT == TDBP_CF16) {
+    typedef _Float16 h2 __attribute__((ext_vector_type(2)));
+    const h2 u0 = __builtin_bit_cast(h2, (uint32_t)v.x);
+    const h2 u1 = __builtin_bit_cast(h2, (uint32_t)v.y);
+    const float w1 = fr * sc, w0 = sc - w1;
+    return make_float2(fmaf((float)u1.x, w1, fmaf((float)u0.x, w0, 0.0f)),
+                       fmaf((float)u1.y, w1, fmaf((float)u0.y, w0, 0.0f)));
+  } else {
+    const float w1 = fr * sc, w0 = sc - w1;
+    return make_float2(
+        fmaf(__uint_as_float(v.y << 16), w1, __uint_as_float(v.x << 16) * w0),
+        fmaf(__uint_as_float(v.y & 0xffff0000u), w1,
+             __uint_as_float(v.x & 0xffff0000u) * w0));
+  }
+}
+
+// Per-lane constants of the mixed pulse loop.  With d = x - x0, aref =
+// u_ref.d, bb = |d|^2 and w = 1 - aref/R0 + (aref/R0)^2 (1/(R0+a) ~ w/R0):
+// the range excess over R0 + aref is
+//   sd = da + w (bb - a^2) / (2 R0),   a = aref + da,
+// and w (bb - a^2) = t0 - (w da + t1) da with the per-pixel t0 = w (bb -
+// aref^2), t1 = 2 w aref (both formed in FP64), so sd takes three FMAs.
 struct MixedPixel {
   float dx, dy, dz;   // x - x0
-  float bb;           // |x - x0|^2
-  float aref;         // u_ref . (x - x0)
+  float t0;           // w (bb - aref^2)
+  float t1;           // 2 w aref
   float arefd;        // aref / dmrg
-  float w;            // 1 - aref/R0 + (aref/R0)^2: 1/(R0+a) ~ w/R0
+  float w;
 };
 
 // One pixel-pulse pair; CHECK selects the per-lane window and gate tests.
@@ -298,10 +332,9 @@ __device__ __forceinline__ void mixed_pair(const MixedPixel& m, const PulseRec& 
                                            bool inwin, unsigned smax, float& sr,
                                            float& si) {
   const float da = fmaf(r.r0.x, m.dx, fmaf(r.r0.y, m.dy, r.r0.z * m.dz));
-  const float a = m.aref + da;
-  const float t = fmaf(-a, a, m.bb);
-  const float del = t * r.r1.z * m.w;
-  const float sd = da + del;                      // R - R0 - aref
+  const float e = fmaf(m.w, da, m.t1);
+  const float t = fmaf(-e, da, m.t0);             // w (bb - a^2)
+  const float sd = fmaf(t, r.r1.z, da);           // R - R0 - aref
   const float rho = fmaf(sd, invd, r.r1.x + m.arefd);
   int il;  // floor(rho) as an integer in one instruction
   asm("v_cvt_flr_i32_f32 %0, %1" : "=v"(il) : "v"(rho));
@@ -320,6 +353,48 @@ __device__ __forceinline__ void mixed_pair(const MixedPixel& m, const PulseRec& 
   si = fmaf(c.x, sn, si);
   si = fmaf(c.y, cs, si);
 }
+
+// The test-free pair split in two for a one-pulse software pipeline: the
+// front computes the range, issues the sample load and the phase argument;
+// the back (a pulse later) lerps and accumulates.  Same arithmetic as
+// mixed_pair<DT, false>.
+template <int DT>
+struct PairFront {
+  typename RawPair<DT>::type v;
+  float fr, cyc, sc;
+};
+template <int DT>
+__device__ __forceinline__ PairFront<DT> mixed_front(const MixedPixel& m, const PulseRec& r,
+                                                     grow_t rowb, float Kf, float invd) {
+  const float da = fmaf(r.r0.x, m.dx, fmaf(r.r0.y, m.dy, r.r0.z * m.dz));
+  const float e = fmaf(m.w, da, m.t1);
+  const float t = fmaf(-e, da, m.t0);
+  const float sd = fmaf(t, r.r1.z, da);
+  const float rho = fmaf(sd, invd, r.r1.x + m.arefd);
+  int il;
+  asm("v_cvt_flr_i32_f32 %0, %1" : "=v"(il) : "v"(rho));
+  const unsigned off =
+      (unsigned)il * (unsigned)elem_bytes<DT>() + (unsigned)__float_as_int(r.r1.y);
+  PairFront<DT> f;
+  f.v = gload<typename RawPair<DT>::type>(rowb + off);
+  f.fr = __builtin_amdgcn_fractf(rho);
+  f.cyc = fmaf(Kf, sd, r.r0.w);
+  f.sc = r.r1.w;
+  return f;
+}
+template <int DT>
+__device__ __forceinline__ void mixed_back(const PairFront<DT>& f, float& sr, float& si) {
+  const float2 c = lerp_raw<DT>(f.v, f.fr, f.sc);
+  const float sn = __builtin_amdgcn_sinf(f.cyc), cs = __builtin_amdgcn_cosf(f.cyc);
+  sr = fmaf(c.x, cs, sr);
+  sr = fmaf(-c.y, sn, sr);
+  si = fmaf(c.x, sn, si);
+  si = fmaf(c.y, cs, si);
+}
+
+#ifndef TDBP_PIPELINE
+#define TDBP_PIPELINE 1
+#endif
 
 template <int TR, bool STACK, int DT>
 __device__ __forceinline__ void mixed_tile(const BurstDev& b, int tile,
@@ -384,14 +459,15 @@ __device__ __forceinline__ void mixed_tile(const BurstDev& b, int tile,
   const double bb64 = d64[0] * d64[0] + d64[1] * d64[1] + d64[2] * d64[2];
   const double aref64 = uref[0] * d64[0] + uref[1] * d64[1] + uref[2] * d64[2];
   const double q = aref64 / R0ref;
+  const double w64 = 1.0 - q + q * q;
   MixedPixel m;
   m.dx = (float)d64[0];
   m.dy = (float)d64[1];
   m.dz = (float)d64[2];
-  m.bb = (float)bb64;
-  m.aref = (float)aref64;
+  m.t0 = (float)(w64 * (bb64 - aref64 * aref64));
+  m.t1 = (float)(2.0 * w64 * aref64);
   m.arefd = (float)(aref64 * b.inv_dmrg);
-  m.w = (float)(1.0 - q + q * q);
+  m.w = (float)w64;
   const double cp64 = aref64 * b.two_over_wvl;
   const float cpix = (float)(cp64 - rint(cp64));
   if (live) atomicMax(&sh->dmax, __float_as_uint((float)sqrt(bb64)));
@@ -460,6 +536,34 @@ __device__ __forceinline__ void mixed_tile(const BurstDev& b, int tile,
         // Every pulse of the stage keeps the whole tile inside the gate and
         // every live lane of the wave is inside its window: no tests.
         int qq = ca;
+#if TDBP_PIPELINE
+        // Two pairs in flight: each sample load is issued a whole pair ahead
+        // of its use (fa and fb keep their own registers).  An odd count
+        // starts with one plain pair; an empty core (cb <= ca) does nothing.
+        if (qq < cb && ((cb - qq) & 1)) {
+          mixed_pair<DT, false>(m, rec[qq - p0], base + (size_t)qq * rowbytes, Kf, invd, true,
+                                smax, sr, si);
+          ++qq;
+        }
+        if (qq < cb) {
+          grow_t rp = base + (size_t)qq * rowbytes;
+          const PulseRec* rr = rec + (qq - p0);
+          PairFront<DT> fa = mixed_front<DT>(m, rr[0], rp, Kf, invd);
+          PairFront<DT> fb;
+          for (; qq + 2 < cb; qq += 2) {
+            fb = mixed_front<DT>(m, rr[1], rp + rowbytes, Kf, invd);
+            mixed_back<DT>(fa, sr, si);
+            rp += 2 * rowbytes;
+            rr += 2;
+            fa = mixed_front<DT>(m, rr[0], rp, Kf, invd);
+            mixed_back<DT>(fb, sr, si);
+          }
+          fb = mixed_front<DT>(m, rr[1], rp + rowbytes, Kf, invd);
+          mixed_back<DT>(fa, sr, si);
+          mixed_back<DT>(fb, sr, si);
+          qq = cb;
+        }
+#endif
 #if TDBP_CORE_UNROLL > 1
         for (; qq < cb - 1; qq += 2) {
           mixed_pair<DT, false>(m, rec[qq - p0], base + (size_t)qq * rowbytes, Kf, invd,
