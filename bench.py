@@ -580,6 +580,8 @@ def main():
     ap.add_argument("--storage", default="cf16", choices=["cf16", "cbf16", "cf32"])
     ap.add_argument("--bursts", type=int, default=27)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--variant", type=int, default=-1,
+                    help="kernel variant of the batched mixed run (default: autotune; 8 = MFMA)")
     ap.add_argument("--plain-passes", type=int, default=2)
     ap.add_argument("--workload", default="scene",
                     choices=["scene", "burst", "small", "stack", "dropin", "rc"],
@@ -666,12 +668,17 @@ def main():
         batch.close()
         return
 
-    # Autotune the mixed kernel's tile shape on the largest burst.
-    big = max(dev, key=lambda d: d["sc"].geom.nlines * d["sc"].geom.demwidth)
-    best, tune_ms = T.autotune(big["sc"].geom, smp(big, stor), big["sat"], big["dem"],
-                               big["out"], T.MIXED, reps=2)
-    log(f"autotune: variant {best} {T.variant_info(best)}; "
-        f"ms per variant {[round(x, 2) for x in tune_ms]}")
+    # Autotune the mixed kernel's tile shape on the largest burst (or take
+    # the variant given).
+    if args.variant >= 0:
+        T.lib().tdbp_set_variant(args.variant)
+        log(f"variant {args.variant} {T.variant_info(args.variant)} (given)")
+    else:
+        big = max(dev, key=lambda d: d["sc"].geom.nlines * d["sc"].geom.demwidth)
+        best, tune_ms = T.autotune(big["sc"].geom, smp(big, stor), big["sat"], big["dem"],
+                                   big["out"], T.MIXED, reps=2)
+        log(f"autotune: variant {best} {T.variant_info(best)}; "
+            f"ms per variant {[round(x, 2) for x in tune_ms]}")
 
     ev0, ev1 = T.Event(), T.Event()
     stream = T.Stream()

@@ -88,6 +88,9 @@ int build(tdbp_batch* b) {
                       "batch stream")))
     return st;
   if (b->variant < 0) b->variant = default_variant(b->precision);
+  if (b->variant == 8)   // the MFMA kernel needs 32-bit sample offsets in every burst
+    for (auto& bd : b->bursts)
+      if (!mfma_fits(bd)) b->variant = 5;
   b->max_lines = 0;
   for (auto& bd : b->bursts) {
     set_tiles(&bd, b->variant);

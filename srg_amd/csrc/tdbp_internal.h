@@ -44,8 +44,14 @@ struct BurstDev {
 struct Variant {
   int tile_rows;     // rows of a wave tile
   int stack_rows;    // 1: waves stacked along rows, 0: along columns
+  int mfma;          // 1: the MFMA kernel (16 waves on a 16 x 16 tile; the
+                     //    tile shape is that of tile_rows 4, stacked)
 };
-constexpr int kNumVariants = 8;
+constexpr int kNumVariants = 9;
+// The MFMA kernel addresses a burst's samples with 32-bit byte offsets.
+inline bool mfma_fits(const BurstDev& b) {
+  return (size_t)b.L * (size_t)b.S * (b.dtype == TDBP_CF32 ? 8u : 4u) < (size_t)0xffffffffu;
+}
 extern const Variant kVariants[kNumVariants];
 
 inline int wg_rows(const Variant& v) {

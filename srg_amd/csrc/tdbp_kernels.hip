@@ -39,7 +39,8 @@
 namespace tdbp {
 
 const Variant kVariants[kNumVariants] = {
-    {8, 0}, {4, 0}, {2, 0}, {1, 0}, {8, 1}, {4, 1}, {2, 1}, {1, 1}};
+    {8, 0, 0}, {4, 0, 0}, {2, 0, 0}, {1, 0, 0}, {8, 1, 0},
+    {4, 1, 0}, {2, 1, 0}, {1, 1, 0}, {4, 1, 1}};
 
 typedef const double __attribute__((address_space(4)))* const_dptr;
 
@@ -661,6 +662,454 @@ __device__ __forceinline__ void mixed_tile(const BurstDev& b, int tile,
   store_out(b, p, fr_ * pcs - fi_ * ps, fr_ * ps + fi_ * pcs);
 }
 
+// ------------------------------------- mixed precision on the matrix cores
+//
+// The same pair arithmetic as mixed_tile, with the range chain on MFMA.  With
+// da = du.d, the range index and phase of a pixel-pulse pair are
+//   sd  = da + r1z t0 - r1z t1 da - r1z w da^2          (R - R0 - aref)
+//   rho = f0 + arefd + sd/dmrg,      cyc = c0 + (2/lambda) sd
+// -- linear forms in per-pixel features F (d_i, t0, t1 d_i, w d_i d_j, 1,
+// arefd) and per-pulse weights G (du_i, r1z, -r1z du_i, -r1z du_i du_j, f0 or
+// c0, 1).  Every product F G is carried by f16 pieces after a per-tile
+// power-of-two balance that puts both factors near sqrt(|F G|): first-order
+// terms as Fh Gh + Fh Gl + Fl Gh, the second-order ones as Fh Gh, arefd in
+// three pieces.  The products are exact in the FP32 accumulator of
+// v_mfma_f32_16x16x32_f16, so one MFMA per output covers 16 pulses x 16 pixels
+// (K = 32 slots; rho uses 26, cyc 23, the pixel side is shared).  On an IW
+// tile against FP64: rho 7.5e-7 samples RMS, cyc 6.8e-7 rad RMS, the FP32
+// chain's accuracy.  Per pair the VALU keeps only the index, the gather, the
+// lerp, sin/cos and the MAC.
+//
+// Workgroup: 16 waves, wave w on row w of a 16 x 16 pixel tile.  Lane
+// l = 16 kg + n holds column n and, in each 16-pulse step, pulses 4 kg + v
+// (v = 0..3) -- the MFMA's output layout -- so it has four gathers in flight.
+
+constexpr int kMBlock = 512;    // 8 waves, two tile rows each
+constexpr int kMStage = 256;    // pulses per stage
+constexpr int kMRowA = 72;      // halves per pulse row in LDS: 64 used; 144 B keeps
+                                // rows 16-byte aligned and, being an odd number of
+                                // 16-byte units, spreads a quarter-wave's 16-byte
+                                // reads over all banks
+constexpr int kMRowB = 40;      // halves per pixel row in LDS (32 used, padded)
+
+typedef _Float16 h8m __attribute__((ext_vector_type(8)));
+typedef float f4m __attribute__((ext_vector_type(4)));
+
+struct MShared {
+  union {
+    struct {
+      _Float16 pf[256][kMRowB];       // pixel features
+      int2 win[256];                  // pixel pulse window [azs, aze), empty if not live
+    } tl;                             // tile set-up (read into registers before stage 0)
+    struct {
+      _Float16 aw[kMStage][kMRowA];   // pulse weights: rho slots 0-31, cyc 32-63
+      unsigned rowoff[kMStage];       // (q S + i0) * elem: byte offset of sample i0 of row q
+      float sc[kMStage];              // per-pulse sample scale
+      int i0[kMStage];
+    } st;
+    float2 part[4][256];              // per-quarter pixel sums (tile end)
+  } u;
+  double x0[3];
+  double uref[3], R0ref, U;           // tile-uniform, computed once by thread 0
+  double fsc[4], kr[4], kc[4];        // term scales (F side) and weight constants
+  int key, wlo, whi;
+  unsigned dmax;
+};
+
+// f16 pieces are made with v_cvt_pkrtz_f16_f32 (two floats -> two halves,
+// round toward zero, one instruction).  A split x = h + l keeps l = x - h
+// exactly in FP32, so the truncation only moves error into l.
+__device__ __forceinline__ unsigned mpk(float a, float b) {
+  return __builtin_bit_cast(unsigned, __builtin_amdgcn_cvt_pkrtz(a, b));
+}
+__device__ __forceinline__ float mhi(float x) {   // x truncated to f16, as float
+  typedef _Float16 h2 __attribute__((ext_vector_type(2)));
+  return (float)__builtin_bit_cast(h2, mpk(x, 0.f)).x;
+}
+
+// Exponent e of the power of two 2^e that balances a product: mF 2^e ~ mG
+// 2^-e (within a factor of 2: only magnitudes matter).
+__device__ __forceinline__ int mbalance(double mF, double mG) {
+  return (ilogb(fmax(mG, 1e-300)) - ilogb(fmax(mF, 1e-300))) / 2;
+}
+
+// F-side scales of the four term families (the G side takes their inverses).
+struct MScale {
+  double da, t0, t1, q;
+};
+
+// K-slot layout (32 slots = 16 words, pixel side F / pulse side G):
+//   0-7   first-order terms T = da_x, da_y, da_z, t0: F (Fh, Fh), G (Gh, Gl)
+//   8-11  the same terms:                               F Fl,       G Gh
+//   12-14 t1 d_i r1z du_i                               F Fh,       G Gh
+//   15-20 w d_i d_j r1z du_i du_j (xx yy zz xy xz yz)   F Fh,       G Gh
+//   21-22 f0 (rho) or c0 (cyc):                         F 1, 1,     G Gh, Gl
+//   23-25 arefd (rho only):                             F Fh Fm Fl, G 1, 1, 1 (cyc 0)
+//   26-31 zero
+__device__ __forceinline__ void mfma_pixel_features(const double* d, double t0, double t1,
+                                                    double w, double arefd, const MScale& s,
+                                                    unsigned* f) {
+  const float x[4] = {(float)(d[0] * s.da), (float)(d[1] * s.da), (float)(d[2] * s.da),
+                      (float)(t0 * s.t0)};
+  float lo[4];
+#pragma unroll
+  for (int t = 0; t < 4; ++t) {
+    const float h = mhi(x[t]);
+    lo[t] = x[t] - h;
+    f[t] = mpk(x[t], x[t]);
+  }
+  f[4] = mpk(lo[0], lo[1]);
+  f[5] = mpk(lo[2], lo[3]);
+  const double t1s = t1 * s.t1, ws = w * s.q;
+  f[6] = mpk((float)(t1s * d[0]), (float)(t1s * d[1]));
+  f[7] = mpk((float)(t1s * d[2]), (float)(ws * d[0] * d[0]));
+  f[8] = mpk((float)(ws * d[1] * d[1]), (float)(ws * d[2] * d[2]));
+  f[9] = mpk((float)(ws * d[0] * d[1]), (float)(ws * d[0] * d[2]));
+  f[10] = mpk((float)(ws * d[1] * d[2]), 1.f);
+  // arefd in three pieces from FP64 (it needs ~1e-6 sample of ~60)
+  const float ah = mhi((float)arefd);
+  const double ar = arefd - (double)ah;
+  const float am = mhi((float)ar);
+  f[11] = mpk(1.f, ah);
+  f[12] = mpk(am, (float)(ar - (double)am));
+  f[13] = 0u;
+  f[14] = 0u;
+  f[15] = 0u;
+}
+
+// Weights of one output: k = c / scale per term family (c = 1/dmrg for rho,
+// 2/lambda for cyc), each formed in FP64 and rounded once to FP32 (2^-24,
+// below the 2^-22 the split keeps).
+struct MWeightK {
+  double da, t0, t1, q;
+};
+__device__ __forceinline__ void mfma_pulse_weights(const double* du, double r1z, double k0,
+                                                   const MWeightK& k, bool rho,
+                                                   unsigned* g) {
+  const float x[4] = {(float)(du[0] * k.da), (float)(du[1] * k.da), (float)(du[2] * k.da),
+                      (float)(r1z * k.t0)};
+#pragma unroll
+  for (int t = 0; t < 4; ++t) g[t] = mpk(x[t], x[t] - mhi(x[t]));
+  g[4] = mpk(x[0], x[1]);
+  g[5] = mpk(x[2], x[3]);
+  const double r1 = -r1z * k.t1, rq = -r1z * k.q;
+  g[6] = mpk((float)(r1 * du[0]), (float)(r1 * du[1]));
+  g[7] = mpk((float)(r1 * du[2]), (float)(rq * du[0] * du[0]));
+  g[8] = mpk((float)(rq * du[1] * du[1]), (float)(rq * du[2] * du[2]));
+  g[9] = mpk((float)(2.0 * rq * du[0] * du[1]), (float)(2.0 * rq * du[0] * du[2]));
+  const float kf = (float)k0;
+  g[10] = mpk((float)(2.0 * rq * du[1] * du[2]), kf);
+  const float one = rho ? 1.f : 0.f;
+  g[11] = mpk(kf - mhi(kf), one);
+  g[12] = mpk(one, one);
+  g[13] = 0u;
+  g[14] = 0u;
+  g[15] = 0u;
+}
+
+template <int DT>
+__device__ __forceinline__ void mfma_tile(const BurstDev& b, int tile, MShared& sh) {
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, wave = tid >> 6;
+  const int n = lane & 15, kg = lane >> 4;
+  const bool owner = tid < 256;   // owns tile pixel tid (row tid/16, column tid%16)
+
+  PixelCtx p;
+  p.inwin = false;
+  p.computed = false;
+  p.azs = p.aze = 0;
+  p.row = p.col = 0;
+  double x[3] = {0., 0., 0.};
+  bool live = false;
+  if (owner) {
+    p = pixel_of<4, true>(b, tile);   // 16 x 16 tile: row tid/16, column tid%16
+    live = p.aze > p.azs;
+    if (live) pixel_ecef(b, p.row, p.col, x);
+  }
+  if (tid == 0) {
+    sh.key = 0x7fffffff;
+    sh.wlo = 0x7fffffff;
+    sh.whi = -1;
+    sh.dmax = 0u;
+  }
+  __syncthreads();
+  if (live) {
+    const int wr = tid >> 4, wc = tid & 15;
+    const int key = (abs(2 * wr - 16) + abs(2 * wc - 16)) * 1024 + tid;
+    atomicMin(&sh.key, key);
+    atomicMin(&sh.wlo, p.azs);
+    atomicMax(&sh.whi, p.aze);
+  }
+  if (owner) sh.u.tl.win[tid] = live ? make_int2(p.azs, p.aze) : make_int2(0, 0);
+  __syncthreads();
+  const int ref = sh.key;
+  const int wlo = sh.wlo, whi = sh.whi;
+  if (ref == 0x7fffffff) {
+    if (owner) store_out(b, p, 0.f, 0.f);
+    return;
+  }
+  if ((ref & 1023) == tid) {
+    sh.x0[0] = x[0];
+    sh.x0[1] = x[1];
+    sh.x0[2] = x[2];
+  }
+  __syncthreads();
+  const double x0[3] = {uniform(sh.x0[0]), uniform(sh.x0[1]), uniform(sh.x0[2])};
+
+  // Tile-uniform set-up, once (thread 0): look vector and range at the
+  // window's central pulse, |du| at the window's ends.
+  if (tid == 0) {
+    const int pc = (wlo + whi - 1) >> 1;
+    const double* sp = b.satloc + 3 * pc;
+    const double u[3] = {x0[0] - sp[0], x0[1] - sp[1], x0[2] - sp[2]};
+    const double R = sqrt(u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
+    const double inv = 1.0 / R;
+    // |du| over the window ~ half the track span over the range (a
+    // magnitude for the balance only).
+    const double* sl = b.satloc + 3 * wlo;
+    const double* sh_ = b.satloc + 3 * (whi - 1);
+    const double span = fmax(fmax(fabs(sh_[0] - sl[0]), fabs(sh_[1] - sl[1])), fabs(sh_[2] - sl[2]));
+    for (int i = 0; i < 3; ++i) sh.uref[i] = u[i] * inv;
+    sh.R0ref = R;
+    sh.U = fmax(0.5 * span * inv, 1e-12);
+  }
+  __syncthreads();
+  const double uref[3] = {uniform(sh.uref[0]), uniform(sh.uref[1]), uniform(sh.uref[2])};
+  const double R0ref = uniform(sh.R0ref);
+
+  // Per-pixel constants (FP64), tile radius.
+  double d64[3] = {0., 0., 0.};
+  double t0 = 0., t1 = 0., w64 = 1., arefd = 0.;
+  float cpix = 0.f;
+  if (owner) {
+    if (live) {
+      d64[0] = x[0] - x0[0];
+      d64[1] = x[1] - x0[1];
+      d64[2] = x[2] - x0[2];
+    }
+    const double bb = d64[0] * d64[0] + d64[1] * d64[1] + d64[2] * d64[2];
+    const double aref = uref[0] * d64[0] + uref[1] * d64[1] + uref[2] * d64[2];
+    const double q = aref / R0ref;
+    w64 = 1.0 - q + q * q;
+    t0 = w64 * (bb - aref * aref);
+    t1 = 2.0 * w64 * aref;
+    arefd = aref * b.inv_dmrg;
+    const double cp = aref * b.two_over_wvl;
+    cpix = (float)(cp - rint(cp));
+    if (live) atomicMax(&sh.dmax, __float_as_uint((float)sqrt(bb)));
+  }
+  __syncthreads();
+  const double dmax =
+      (double)__uint_as_float((unsigned)__builtin_amdgcn_readfirstlane((int)sh.dmax));
+  // Balance of the four term families for both outputs at once (geometric
+  // mean of 1/dmrg and 2/lambda): the pixel side is shared by the two MFMAs.
+  // Thread 0 computes the scales and the weight constants (c over the scale).
+  if (tid == 0) {
+    const double D = fmax(dmax, 1.0), U = sh.U;
+    const double cg = sqrt(b.inv_dmrg * b.two_over_wvl);
+    const double rz = 0.5 / R0ref;
+    const int e[4] = {mbalance(D, U * cg), mbalance(D * D, cg * rz),
+                      mbalance(2.0 * D * D, U * cg * rz),
+                      mbalance(D * D, 2.0 * U * U * cg * rz)};
+    for (int i = 0; i < 4; ++i) {
+      sh.fsc[i] = ldexp(1.0, e[i]);
+      sh.kr[i] = ldexp(b.inv_dmrg, -e[i]);
+      sh.kc[i] = ldexp(b.two_over_wvl, -e[i]);
+    }
+  }
+  __syncthreads();
+  const MScale sc = {uniform(sh.fsc[0]), uniform(sh.fsc[1]), uniform(sh.fsc[2]),
+                     uniform(sh.fsc[3])};
+  const MWeightK kr = {uniform(sh.kr[0]), uniform(sh.kr[1]), uniform(sh.kr[2]),
+                       uniform(sh.kr[3])};
+  const MWeightK kc = {uniform(sh.kc[0]), uniform(sh.kc[1]), uniform(sh.kc[2]),
+                       uniform(sh.kc[3])};
+  if (owner) {
+    unsigned f[16];
+    mfma_pixel_features(d64, t0, t1, w64, arefd, sc, f);
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+      *(uint4*)&sh.u.tl.pf[tid][8 * k] = make_uint4(f[4 * k], f[4 * k + 1], f[4 * k + 2],
+                                                 f[4 * k + 3]);
+  }
+  __syncthreads();
+  // This lane's pixels: column n of rows 2 wave and 2 wave + 1.
+  const int px0 = (2 * wave) * 16 + n, px1 = px0 + 16;
+  const h8m bf0 = *(const h8m*)&sh.u.tl.pf[px0][8 * kg];
+  const h8m bf1 = *(const h8m*)&sh.u.tl.pf[px1][8 * kg];
+  const int2 wn0 = sh.u.tl.win[px0], wn1 = sh.u.tl.win[px1];
+  const bool lv0 = wn0.y > wn0.x, lv1 = wn1.y > wn1.x;
+  const int qlo = wave_min(min(lv0 ? wn0.x : 0x7fffffff, lv1 ? wn1.x : 0x7fffffff));
+  const int qhi = wave_max(max(lv0 ? wn0.y : -1, lv1 ? wn1.y : -1));
+  const int qc0 = wave_max(max(lv0 ? wn0.x : -1, lv1 ? wn1.x : -1));
+  const int qc1 = wave_min(min(lv0 ? wn0.y : 0x7fffffff, lv1 ? wn1.y : 0x7fffffff));
+  const bool wave_live = qhi > qlo;
+  const grow_t base = (grow_t)b.data;
+  const unsigned smax = (unsigned)(b.S - 2);
+  const unsigned elem = (unsigned)elem_bytes<DT>();
+  const double r_hi = b.r_hi;
+  __syncthreads();   // the stage tables reuse the pixel-feature rows
+
+  float acc_r[2] = {0.f, 0.f}, acc_i[2] = {0.f, 0.f}, cmp_r[2] = {0.f, 0.f},
+        cmp_i[2] = {0.f, 0.f};
+  for (int p0 = wlo; p0 < whi; p0 += kMStage) {
+    // Stage prologue: thread t builds output t >> 8 (0 rho, 1 cyc) of pulse
+    // t & 255 (FP64 geometry, FP32-rounded weights); the rho threads also
+    // write the pulse's scalars and range-gate flag.
+    bool safe_all = true;
+    {
+      const int j = tid & (kMStage - 1);
+      const bool is_rho = tid < kMStage;
+      const int qs = p0 + j;
+      unsigned g[16];
+      if (qs < whi) {
+        const double* sp = b.satloc + 3 * qs;
+        const double u0 = x0[0] - sp[0], u1 = x0[1] - sp[1], u2 = x0[2] - sp[2];
+        const double R0 = sqrt(u0 * u0 + u1 * u1 + u2 * u2);
+        const double inv = 1.0 / R0;
+        const double du[3] = {u0 * inv - uref[0], u1 * inv - uref[1], u2 * inv - uref[2]};
+        const double rb = (R0 - b.rngstart) * b.inv_dmrg;
+        const double i0 = floor(rb);
+        if (is_rho) {
+          mfma_pulse_weights(du, 0.5 * inv, rb - i0, kr, true, g);
+          const double rlo = rb - dmax * b.inv_dmrg - 1.0;
+          const double rhi =
+              rb + (dmax + dmax * dmax / (2.0 * (R0 - dmax))) * b.inv_dmrg + 1.0;
+          safe_all = rlo >= 1.0 && rhi <= b.r_hi - 1.0;
+          sh.u.st.rowoff[j] = (unsigned)(((size_t)qs * (size_t)b.S + (size_t)(long)i0) * elem);
+          sh.u.st.sc[j] = (DT == TDBP_CF32 || b.scale == nullptr) ? 1.f : b.scale[qs];
+          sh.u.st.i0[j] = (int)i0;
+        } else {
+          const double c = R0 * b.two_over_wvl;
+          mfma_pulse_weights(du, 0.5 * inv, c - rint(c), kc, false, g);
+        }
+      } else {
+#pragma unroll
+        for (int k = 0; k < 16; ++k) g[k] = 0u;
+        if (is_rho) {
+          sh.u.st.rowoff[j] = 0u;
+          sh.u.st.sc[j] = 0.f;
+          sh.u.st.i0[j] = -0x40000000;   // fails the gate test
+        }
+      }
+      const int col = is_rho ? 0 : 32;
+#pragma unroll
+      for (int k = 0; k < 4; ++k)
+        *(uint4*)&sh.u.st.aw[j][col + 8 * k] =
+            make_uint4(g[4 * k], g[4 * k + 1], g[4 * k + 2], g[4 * k + 3]);
+    }
+    const int stage_safe = __syncthreads_and(safe_all ? 1 : 0);
+    float sr[2] = {0.f, 0.f}, si[2] = {0.f, 0.f};
+    if (wave_live) {
+      const int qa = max(p0, qlo), qb = min(p0 + kMStage, qhi);
+      for (int s0 = ((qa - p0) >> 4) << 4; p0 + s0 < qb; s0 += 16) {
+        const int q0 = p0 + s0;
+        const h8m ar = *(const h8m*)&sh.u.st.aw[s0 + n][8 * kg];
+        const h8m ac = *(const h8m*)&sh.u.st.aw[s0 + n][32 + 8 * kg];
+        const uint4 ro = *(const uint4*)&sh.u.st.rowoff[s0 + 4 * kg];
+        const float4 s4 = *(const float4*)&sh.u.st.sc[s0 + 4 * kg];
+        const unsigned rov[4] = {ro.x, ro.y, ro.z, ro.w};
+        const float scv[4] = {s4.x, s4.y, s4.z, s4.w};
+        const f4m z = {0.f, 0.f, 0.f, 0.f};
+        const bool core = stage_safe && q0 >= qc0 && q0 + 16 <= qc1;
+        int4 i4 = make_int4(0, 0, 0, 0);
+        if (!core) i4 = *(const int4*)&sh.u.st.i0[s0 + 4 * kg];
+        const int i0v[4] = {i4.x, i4.y, i4.z, i4.w};
+        // Both groups' products first: the second pair of MFMAs runs while
+        // the first group's pairs are finished on the VALU.
+        const f4m drg[2] = {__builtin_amdgcn_mfma_f32_16x16x32_f16(ar, bf0, z, 0, 0, 0),
+                            __builtin_amdgcn_mfma_f32_16x16x32_f16(ar, bf1, z, 0, 0, 0)};
+        const f4m dcg[2] = {__builtin_amdgcn_mfma_f32_16x16x32_f16(ac, bf0, z, 0, 0, 0),
+                            __builtin_amdgcn_mfma_f32_16x16x32_f16(ac, bf1, z, 0, 0, 0)};
+        if (core) {
+          // Every live lane of the wave is inside its windows for the whole
+          // step and the whole tile inside the range gate: no tests.  All
+          // eight gathers are issued before the first is used.
+          typename RawPair<DT>::type v[2][4];
+          float fr[2][4];
+#pragma unroll
+          for (int gi = 0; gi < 2; ++gi)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+              // floor(rho) in one instruction.  The asm takes fr as an input
+              // so it runs after fract, whose read of the MFMA result the
+              // compiler sees and guards with the MFMA-to-VALU wait states.
+              fr[gi][j] = __builtin_amdgcn_fractf(drg[gi][j]);
+              int il;
+              asm("v_cvt_flr_i32_f32 %0, %1" : "=v"(il) : "v"(drg[gi][j]), "v"(fr[gi][j]));
+              v[gi][j] =
+                  gload<typename RawPair<DT>::type>(base + ((unsigned)il * elem + rov[j]));
+            }
+#pragma unroll
+          for (int gi = 0; gi < 2; ++gi)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+              const float2 c = lerp_raw<DT>(v[gi][j], fr[gi][j], scv[j]);
+              const float sn = __builtin_amdgcn_sinf(dcg[gi][j]),
+                          cs = __builtin_amdgcn_cosf(dcg[gi][j]);
+              sr[gi] = fmaf(c.x, cs, sr[gi]);
+              sr[gi] = fmaf(-c.y, sn, sr[gi]);
+              si[gi] = fmaf(c.x, sn, si[gi]);
+              si[gi] = fmaf(c.y, cs, si[gi]);
+            }
+        } else {
+#pragma unroll
+          for (int gi = 0; gi < 2; ++gi) {
+            const f4m dr = drg[gi], dc = dcg[gi];
+            const int2 wn = gi ? wn1 : wn0;
+            const unsigned nwin = (unsigned)(wn.y - wn.x);
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+              const int q = q0 + 4 * kg + j;
+              const float rho = dr[j];
+              const int il = (int)floorf(rho);
+              // gate: window, sample index in [0, S-2], r <= r_hi
+              if ((unsigned)(q - wn.x) < nwin && (unsigned)(il + i0v[j]) <= smax &&
+                  rho <= (float)(r_hi - (double)i0v[j])) {
+                const typename RawPair<DT>::type vv =
+                    gload<typename RawPair<DT>::type>(base + ((unsigned)il * elem + rov[j]));
+                const float2 c = lerp_raw<DT>(vv, __builtin_amdgcn_fractf(rho), scv[j]);
+                const float sn = __builtin_amdgcn_sinf(dc[j]),
+                            cs = __builtin_amdgcn_cosf(dc[j]);
+                sr[gi] = fmaf(c.x, cs, sr[gi]);
+                sr[gi] = fmaf(-c.y, sn, sr[gi]);
+                si[gi] = fmaf(c.x, sn, si[gi]);
+                si[gi] = fmaf(c.y, cs, si[gi]);
+              }
+            }
+          }
+        }
+      }
+    }
+#pragma unroll
+    for (int gi = 0; gi < 2; ++gi) {
+      const float yr = sr[gi] - cmp_r[gi], tr = acc_r[gi] + yr;
+      cmp_r[gi] = (tr - acc_r[gi]) - yr;
+      acc_r[gi] = tr;
+      const float yi = si[gi] - cmp_i[gi], ti = acc_i[gi] + yi;
+      cmp_i[gi] = (ti - acc_i[gi]) - yi;
+      acc_i[gi] = ti;
+    }
+    __syncthreads();
+  }
+  // Quarter sums per pixel, joined by the pixel's owner, then the per-pixel
+  // phase offset frac(2/lambda u_ref.d).
+  sh.u.part[kg][px0] = make_float2(acc_r[0] - cmp_r[0], acc_i[0] - cmp_i[0]);
+  sh.u.part[kg][px1] = make_float2(acc_r[1] - cmp_r[1], acc_i[1] - cmp_i[1]);
+  __syncthreads();
+  if (owner) {
+    float fr_ = 0.f, fi_ = 0.f;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      fr_ += sh.u.part[k][tid].x;
+      fi_ += sh.u.part[k][tid].y;
+    }
+    float ps, pcs;
+    sincospif(2.0f * cpix, &ps, &pcs);
+    store_out(b, p, fr_ * pcs - fi_ * ps, fr_ * ps + fi_ * pcs);
+  }
+}
+
 // ----------------------------------------------------------- kernels
 
 template <int TR, bool STACK, bool CPU>
@@ -683,6 +1132,42 @@ __global__ __launch_bounds__(kBlock) TDBP_MIXED_OCC void k_mixed(BurstDev b) {
   __shared__ TileShared sh;
   const int nb = b.tiles_r * b.tiles_c;
   mixed_tile<TR, STACK, DT>(b, xcd_tile(blockIdx.x, nb), rec, &sh);
+}
+
+// MFMA variant of k_mixed (512-thread workgroups, 16 x 16 pixel tiles).
+template <int DT>
+__global__ __launch_bounds__(kMBlock) TDBP_MIXED_OCC void k_mixed_mfma(BurstDev b) {
+  __shared__ MShared sh;
+  const int nb = b.tiles_r * b.tiles_c;
+  mfma_tile<DT>(b, xcd_tile(blockIdx.x, nb), sh);
+}
+
+// Persistent batch kernel on the MFMA tile; same queues as k_batch_mixed.
+template <int DT>
+__global__ __launch_bounds__(kMBlock) TDBP_MIXED_OCC void k_batch_mfma(
+    const BurstDev* __restrict__ bursts, const int2* __restrict__ items,
+    const int* __restrict__ qbase, int* counters) {
+  __shared__ MShared sh;
+  __shared__ int item_s;
+  const int home = blockIdx.x & 7;
+  int qi = 0;
+  while (qi < 8) {
+    const int qk = (home + qi) & 7;
+    if (threadIdx.x == 0) {
+      const int k = atomicAdd(&counters[qk], 1);
+      item_s = (k < qbase[qk + 1] - qbase[qk]) ? qbase[qk] + k : -1;
+    }
+    __syncthreads();
+    const int it = item_s;
+    __syncthreads();
+    if (it < 0) {
+      ++qi;
+      continue;
+    }
+    const int2 w = items[it];
+    mfma_tile<DT>(bursts[w.x], w.y, sh);
+    __syncthreads();
+  }
 }
 
 // Persistent batch kernel: workgroup on XCD x = blockIdx % 8 drains queue x,
@@ -1189,6 +1674,7 @@ struct OccRun {
 
 hipError_t launch_plain(const BurstDev& b, int variant, hipStream_t s) {
   switch (variant) {
+    case 8: return plain_launch<4, true>(b, s);   // same tile as the MFMA variant
     case 0: return plain_launch<8, false>(b, s);
     case 1: return plain_launch<4, false>(b, s);
     case 2: return plain_launch<2, false>(b, s);
@@ -1200,7 +1686,20 @@ hipError_t launch_plain(const BurstDev& b, int variant, hipStream_t s) {
   }
 }
 
+template <int DT>
+hipError_t mfma_launch(const BurstDev& b, hipStream_t s) {
+  const int nb = b.tiles_r * b.tiles_c;
+  hipLaunchKernelGGL((k_mixed_mfma<DT>), dim3(nb), dim3(kMBlock), 0, s, b);
+  return hipGetLastError();
+}
+
 hipError_t launch_mixed(const BurstDev& b, int variant, hipStream_t s) {
+  if (variant == 8) {
+    if (!mfma_fits(b)) return mixed_dt<4, true>(b, s);   // same tile shape
+    if (b.dtype == TDBP_CF32) return mfma_launch<TDBP_CF32>(b, s);
+    if (b.dtype == TDBP_CF16) return mfma_launch<TDBP_CF16>(b, s);
+    return mfma_launch<TDBP_CBF16>(b, s);
+  }
   switch (variant) {
     case 0: return mixed_dt<8, false>(b, s);
     case 1: return mixed_dt<4, false>(b, s);
@@ -1213,14 +1712,38 @@ hipError_t launch_mixed(const BurstDev& b, int variant, hipStream_t s) {
   }
 }
 
+template <int DT>
+int mfma_occupancy() {
+  int n = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_batch_mfma<DT>, kMBlock, 0) !=
+      hipSuccess)
+    return 1;
+  return n > 0 ? n : 1;
+}
+
 int batch_blocks_per_cu(int precision, int variant, int dtype) {
-  return dispatch<OccRun>(precision, variant, dtype);
+  if (variant == 8 && precision == TDBP_MIXED) {
+    if (dtype == TDBP_CF32) return mfma_occupancy<TDBP_CF32>();
+    if (dtype == TDBP_CF16) return mfma_occupancy<TDBP_CF16>();
+    return mfma_occupancy<TDBP_CBF16>();
+  }
+  return dispatch<OccRun>(precision, variant == 8 ? 5 : variant, dtype);
 }
 
 hipError_t launch_batch(const BurstDev* d_bursts, const int2* d_items,
                         const int* d_qbase, int* d_counters, int precision,
                         int variant, int dtype, int grid, hipStream_t s) {
-  return dispatch<BatchRun>(precision, variant, dtype, d_bursts, d_items,
+  if (variant == 8 && precision == TDBP_MIXED) {
+#define TDBP_M(DT)                                                                    \
+  hipLaunchKernelGGL((k_batch_mfma<DT>), dim3(grid), dim3(kMBlock), 0, s, d_bursts, \
+                     d_items, d_qbase, d_counters);                                  \
+  return hipGetLastError();
+    if (dtype == TDBP_CF32) { TDBP_M(TDBP_CF32) }
+    if (dtype == TDBP_CF16) { TDBP_M(TDBP_CF16) }
+    TDBP_M(TDBP_CBF16)
+#undef TDBP_M
+  }
+  return dispatch<BatchRun>(precision, variant == 8 ? 5 : variant, dtype, d_bursts, d_items,
                             d_qbase, d_counters, grid, s);
 }
 

@@ -156,6 +156,31 @@ def test_mixed_matches_oracle(scene, dtype, request):
     assert np.all(got[~m] == 0)
 
 
+MFMA_VARIANT = 8   # the MFMA kernel: range index and phase from v_mfma_f32_16x16x32_f16
+
+
+@pytest.mark.parametrize("scene", ["c1", "fbm_small"])
+@pytest.mark.parametrize("dtype", [T.CF32, T.CF16, T.CBF16])
+def test_mfma_variant_matches_oracle(scene, dtype, request):
+    """The MFMA kernel to the same bounds as the FP32 chain (its f16-split
+    range index and phase are as accurate as the FP32 chain's), and within
+    1e-5 rad / 1e-5 of the FP32-chain kernel on cf32 samples."""
+    d, ref = request.getfixturevalue(scene)
+    d.fit()
+    got = d.run(T.MIXED, dtype, variant=MFMA_VARIANT)
+    m = ref["azoff"] > 0
+    ph, amp = errors(got, ref["out"], m)
+    print(f"{scene} MFMA {T.DTYPE_NAMES[dtype]} vs oracle: phase {ph*1e3:.4f} mrad, "
+          f"amp {amp:.3e}")
+    pb, ab = BOUNDS[dtype]
+    assert ph <= pb and amp <= ab
+    assert np.all(got[~m] == 0)
+    if dtype == T.CF32:
+        chain = d.run(T.MIXED, dtype, variant=5)
+        ph, amp = errors(got, chain, m)
+        assert ph <= 1e-5 and amp <= 1e-5, (ph, amp)
+
+
 def test_all_mixed_variants_agree(c1):
     d, ref = c1
     d.fit()
@@ -177,8 +202,10 @@ def test_plain_is_deterministic(fbm_small):
     assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
 
 
-@pytest.mark.parametrize("precision,dtype", [(T.MIXED, T.CF16), (T.PLAIN_FP64, T.CF32)])
-def test_batch_equals_single_bursts(gpu, precision, dtype):
+@pytest.mark.parametrize("precision,dtype,v", [(T.MIXED, T.CF16, 4), (T.PLAIN_FP64, T.CF32, 4),
+                                               (T.MIXED, T.CF16, MFMA_VARIANT),
+                                               (T.MIXED, T.CBF16, MFMA_VARIANT)])
+def test_batch_equals_single_bursts(gpu, precision, dtype, v):
     """Uneven bursts in one persistent launch (graph-replayed three times)
     give bit-identical results to one launch per burst."""
     src = synth.fbm_surface((256, 256), seed=4)
@@ -187,7 +214,6 @@ def test_batch_equals_single_bursts(gpu, precision, dtype):
               for i, (L, S, W, nl) in enumerate([(400, 3000, 300, 120), (520, 4100, 170, 90),
                                                  (450, 2500, 260, 150)])]
     devs = [Dev(sc) for sc in scenes]
-    v = 4
     singles = []
     for d in devs:
         d.fit()
