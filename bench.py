@@ -399,6 +399,37 @@ def rc_bench(args, synth, T):
     ms_both = timed(both, max(2, args.steps // 2))
     ms_bp = timed(tdbp_, args.steps)
     ms_e2e = timed(lambda: (packed(), tdbp_()), args.steps)
+
+    # A stream of bursts: range compression of burst k+1 (HBM-bound) on one
+    # stream while burst k is backprojected (VALU-bound) on the other.  Each
+    # stream has its own packed buffer and image; wall time over nburst bursts.
+    pw2 = T.DeviceArray((g.L, S), np.uint32)
+    ps2 = T.DeviceArray((g.L,), np.float32)
+    out2 = T.DeviceArray((g.nlines, g.demwidth), np.complex64)
+    stream2 = T.Stream()
+    bufs = [(stream, pw, ps, out), (stream2, pw2, ps2, out2)]
+
+    def burst_stream(nburst, nstreams):
+        for k in range(nburst):
+            s_, w_, sc_, o_ = bufs[k % nstreams]
+            T.range_compress(d_raw, c, None, packed=(T.CF16, w_, sc_), stream=s_)
+            T.backproject(g, T.samples(w_, T.CF16, sc_), d_sat, d_dem, o_, T.MIXED,
+                          d_coef=d_coef, stream=s_)
+
+    nburst = 12
+    ms_stream = {}
+    for nst in (1, 2):
+        best = 1e30
+        for _ in range(3):
+            T.sync()
+            t0 = time.perf_counter()
+            burst_stream(nburst, nst)
+            stream.sync()
+            stream2.sync()
+            best = min(best, (time.perf_counter() - t0) * 1e3 / nburst)
+        ms_stream[nst] = best
+    log(f"burst stream of {nburst}: {ms_stream[1]:.3f} ms/burst on one stream, "
+        f"{ms_stream[2]:.3f} on two (compression overlapping backprojection)")
     alg = g.L * ((c.nsamps + 10) * 8 + S * 4 + 4)     # read echoes, write packed + scale
     gbps = alg / (ms_rc * 1e-3) / 1e9
     img = out.download()
@@ -424,6 +455,10 @@ def rc_bench(args, synth, T):
             "ms_with_cf32_copy": round(ms_both, 4), "ms_backprojection": round(ms_bp, 4),
             "ms_raw_to_image": round(ms_e2e, 4),
             "raw_to_image_gpps": round(pairs / ms_e2e / 1e6, 2),
+            "burst_stream": {"bursts": nburst,
+                             "ms_per_burst_one_stream": round(ms_stream[1], 4),
+                             "ms_per_burst_two_streams": round(ms_stream[2], 4),
+                             "gpps_two_streams": round(pairs / ms_stream[2] / 1e6, 2)},
             "target_phase_err_rad": [round(float(x), 5) for x in foc],
             "cpu_baseline": cpu}
     print(json.dumps(line), flush=True)

@@ -178,6 +178,8 @@ int tdbp_event_create(void **event);
 int tdbp_event_destroy(void *event);
 int tdbp_event_record(void *event, void *stream);
 int tdbp_event_elapsed_ms(void *start, void *stop, float *ms);
+/* Later work on `stream` waits for `event` (recorded on another stream). */
+int tdbp_stream_wait_event(void *stream, void *event);
 /* HBM bytes free / total on the current device. */
 int tdbp_mem_info(size_t *free_bytes, size_t *total_bytes);
 

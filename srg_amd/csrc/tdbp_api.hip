@@ -421,6 +421,10 @@ int tdbp_event_record(void* e, void* s) {
   return hip_check(hipEventRecord((hipEvent_t)e, (hipStream_t)s),
                    "hipEventRecord");
 }
+int tdbp_stream_wait_event(void* s, void* e) {
+  return hip_check(hipStreamWaitEvent((hipStream_t)s, (hipEvent_t)e, 0), "hipStreamWaitEvent");
+}
+
 int tdbp_event_elapsed_ms(void* a, void* b, float* ms) {
   return hip_check(hipEventElapsedTime(ms, (hipEvent_t)a, (hipEvent_t)b),
                    "hipEventElapsedTime");

@@ -1,8 +1,9 @@
 // tdbp_fft.hip -- cached rocFFT plans for the batched 1-D complex FP32
 // transforms of range compression and interferogram formation.  One plan per
-// (device, length, batch, direction), created on first use with its own
-// execution info and work buffer; every use holds the module mutex while it
-// sets the stream and enqueues (rocfft_execute is asynchronous).
+// (device, stream, length, batch, direction), created on first use with its
+// own execution info and work buffer -- a plan's work buffer must not be
+// shared by transforms running concurrently on two streams; every use holds
+// the module mutex while it enqueues (rocfft_execute is asynchronous).
 #include <hip/hip_runtime.h>
 #include <rocfft/rocfft.h>
 
@@ -16,6 +17,7 @@ namespace tdbp {
 
 struct FftPlan {
   int dev;
+  hipStream_t stream;
   size_t len;
   int batch;
   bool inverse;
@@ -77,9 +79,11 @@ int fft_run(size_t len, int batch, bool inverse, void* data, hipStream_t s) {
   std::lock_guard<std::mutex> lk(g_fft_mu);
   FftPlan* p = nullptr;
   for (FftPlan* q : g_plans)
-    if (q->dev == dev && q->len == len && q->batch == batch && q->inverse == inverse) p = q;
+    if (q->dev == dev && q->stream == s && q->len == len && q->batch == batch &&
+        q->inverse == inverse)
+      p = q;
   if (!p) {
-    p = new FftPlan{dev, len, batch, inverse};
+    p = new FftPlan{dev, s, len, batch, inverse};
     const int st = make_plan(p);
     if (st) {
       drop_plan(p);

@@ -22,6 +22,7 @@
 #include <cmath>
 #include <cstring>
 #include <mutex>
+#include <vector>
 
 #include "tdbp_api_internal.h"
 #include "tdbp_fft.h"
@@ -157,9 +158,13 @@ k_rc_store(const float2* __restrict__ raw, int nlines, int stride, int base,
 
 // ------------------------------------------------------------ state
 
-// Per device: the transform buffer of one batch of lines and the reference
-// spectrum of the last chirp seen.
+// Per (device, stream): the transform buffer of one batch of lines and the
+// reference spectrum of the last chirp seen.  Calls on different streams
+// have their own, so two bursts can be range-compressed concurrently (e.g.
+// one burst's compression overlapping the previous burst's backprojection).
 struct RcState {
+  int dev = -1;
+  hipStream_t stream = nullptr;
   float2* buf = nullptr;   // [kChunk][n], n <= kFftLen
   float2* ref = nullptr;   // [n] reference spectrum for ref_of at length ref_n
   int ref_n = 0;
@@ -168,19 +173,28 @@ struct RcState {
 };
 
 std::mutex g_rc_mu;
-RcState g_rc[kMaxDev];
+std::vector<RcState*> g_rc;
 
-int state_for(int dev, RcState** out) {
-  if (dev < 0 || dev >= kMaxDev) return set_error(TDBP_ERR_ARG, "device %d", dev);
-  RcState& r = g_rc[dev];
-  *out = &r;
-  if (r.buf) return TDBP_OK;
-  if (hipMalloc((void**)&r.ref, sizeof(float2) * (size_t)kFftLen) != hipSuccess)
-    return set_error(TDBP_ERR_NOMEM, "range compression buffers");
-  if (hipMalloc((void**)&r.buf, sizeof(float2) * (size_t)kChunk * kFftLen) != hipSuccess) {
-    r.buf = nullptr;
+int state_for(int dev, hipStream_t stream, RcState** out) {
+  for (RcState* r : g_rc)
+    if (r->dev == dev && r->stream == stream) {
+      *out = r;
+      return TDBP_OK;
+    }
+  RcState* r = new RcState;
+  r->dev = dev;
+  r->stream = stream;
+  if (hipMalloc((void**)&r->ref, sizeof(float2) * (size_t)kFftLen) != hipSuccess) {
+    delete r;
     return set_error(TDBP_ERR_NOMEM, "range compression buffers");
   }
+  if (hipMalloc((void**)&r->buf, sizeof(float2) * (size_t)kChunk * kFftLen) != hipSuccess) {
+    (void)hipFree(r->ref);
+    delete r;
+    return set_error(TDBP_ERR_NOMEM, "range compression buffers");
+  }
+  g_rc.push_back(r);
+  *out = r;
   return TDBP_OK;
 }
 
@@ -253,8 +267,8 @@ int tdbp_range_compress(const void* d_raw, int nlines, int raw_stride, const tdb
   (void)hipGetDevice(&dev);
   std::lock_guard<std::mutex> lk(g_rc_mu);
   RcState* r = nullptr;
-  if ((st = state_for(dev, &r))) return st;
   hipStream_t s = (hipStream_t)stream;
+  if ((st = state_for(dev, s, &r))) return st;
   const int n = transform_length(c->nsamps);
   const unsigned nb = (unsigned)((n + 255) / 256);
   if (!r->ref_valid || !same_chirp(r->ref_of, *c) || r->ref_n != n) {

@@ -66,7 +66,7 @@ def process_raw_burst(d_lines: T.DeviceArray, timeorbit, xx, vv, dem: DemGrid,
     S = chirp.nvalid + 10
     d_sat, d_vel, _ = T.pulse_orbit(d_lines, timeorbit, xx, vv, quantize=quantize_positions,
                                     stream=stream)
-    vmid = d_vel.rows(L // 2 - 1, 1).download()[0]        # satvel(:, rawdatalines/2)
+    vmid = d_vel.rows(L // 2 - 1, 1).download(stream)[0]  # satvel(:, rawdatalines/2)
     geom, ll, ins = T.burst_setup(headers, S, vmid, timeorbit, xx, vv, dem.width, dem.length,
                                   dem.firstlat, dem.firstlon, dem.deltalat, dem.deltalon)
     res = BurstResult(geom=geom, latlons=ll, in_scene=ins, chirp=chirp, d_satloc=d_sat)
@@ -91,3 +91,33 @@ def process_raw_burst(d_lines: T.DeviceArray, timeorbit, xx, vv, dem: DemGrid,
     res.d_out = out
     res._keep = keep
     return res
+
+
+def process_raw_bursts(bursts, timeorbit, xx, vv, dem: DemGrid,
+                       d_grid: Optional[T.DeviceArray] = None, precision: int = T.MIXED,
+                       dtype: int = T.CF16, variant: Optional[int] = None,
+                       quantize_positions: bool = True, streams: int = 2) -> list:
+    """process_raw_burst over several bursts (device arrays of decoded lines),
+    alternating between `streams` streams: a burst's range compression and
+    set-up (HBM-bound) run while the previous burst's backprojection
+    (VALU-bound) runs.  Merges into d_grid keep burst order -- each waits for
+    the previous burst's merge -- so overlapping rows end with the later
+    burst, as one stream would leave them.  Returns the BurstResults after
+    every stream is done."""
+    ss = [T.Stream() for _ in range(max(1, streams))]
+    prev = None
+    out = []
+    for k, d_lines in enumerate(bursts):
+        s = ss[k % len(ss)]
+        res = process_raw_burst(d_lines, timeorbit, xx, vv, dem, None, precision, dtype,
+                                variant, quantize_positions, stream=s)
+        if d_grid is not None and res.d_out is not None:
+            if prev is not None:
+                s.wait(prev)
+            T.merge_geo(d_grid, res.geom.firstline, res.d_out, stream=s)
+            prev = T.Event()
+            prev.record(s)
+        out.append(res)
+    for s in ss:
+        s.sync()
+    return out

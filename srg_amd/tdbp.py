@@ -129,6 +129,7 @@ def _sig(lib):
         "tdbp_stream_create": (C.c_int, [C.POINTER(_VP)]),
         "tdbp_stream_destroy": (C.c_int, [_VP]),
         "tdbp_stream_sync": (C.c_int, [_VP]),
+        "tdbp_stream_wait_event": (C.c_int, [_VP, _VP]),
         "tdbp_device_sync": (C.c_int, []),
         "tdbp_event_create": (C.c_int, [C.POINTER(_VP)]),
         "tdbp_event_destroy": (C.c_int, [_VP]),
@@ -204,7 +205,7 @@ def exported_symbols():
         "tdbp_last_error", "tdbp_device_count", "tdbp_init", "tdbp_dropin_status", "tdbp_dropin_flush",
         "tdbp_dev_alloc", "tdbp_dev_free", "tdbp_memcpy_htod", "tdbp_memcpy_dtoh",
         "tdbp_memcpy_dtod", "tdbp_dev_zero", "tdbp_dev_copy", "tdbp_stream_create", "tdbp_stream_destroy",
-        "tdbp_stream_sync", "tdbp_device_sync", "tdbp_event_create", "tdbp_event_destroy",
+        "tdbp_stream_sync", "tdbp_stream_wait_event", "tdbp_device_sync", "tdbp_event_create", "tdbp_event_destroy",
         "tdbp_event_record", "tdbp_event_elapsed_ms", "tdbp_mem_info", "tdbp_window",
         "tdbp_orbit_interp", "tdbp_setxyz", "tdbp_fit_aperture", "tdbp_setazoff",
         "tdbp_pair_count", "tdbp_pack_samples", "tdbp_chirp_decode", "tdbp_range_compress", "tdbp_synth_raw", "tdbp_crossmul",
@@ -276,21 +277,21 @@ class DeviceArray:
     def from_host(cls, a: np.ndarray, stream=None):
         a = np.ascontiguousarray(a)
         d = cls(a.shape, a.dtype)
-        check(lib().tdbp_memcpy_htod(d.ptr, a.ctypes.data, a.nbytes, stream))
+        check(lib().tdbp_memcpy_htod(d.ptr, a.ctypes.data, a.nbytes, _s(stream)))
         return d
 
     def upload(self, a: np.ndarray, stream=None):
         a = np.ascontiguousarray(a, dtype=self.dtype)
         assert a.nbytes == self.nbytes
-        check(lib().tdbp_memcpy_htod(self.ptr, a.ctypes.data, a.nbytes, stream))
+        check(lib().tdbp_memcpy_htod(self.ptr, a.ctypes.data, a.nbytes, _s(stream)))
 
     def download(self, stream=None) -> np.ndarray:
         out = np.empty(self.shape, self.dtype)
-        check(lib().tdbp_memcpy_dtoh(out.ctypes.data, self.ptr, self.nbytes, stream))
+        check(lib().tdbp_memcpy_dtoh(out.ctypes.data, self.ptr, self.nbytes, _s(stream)))
         return out
 
     def zero(self, stream=None):
-        check(lib().tdbp_dev_zero(self.ptr, self.nbytes, stream))
+        check(lib().tdbp_dev_zero(self.ptr, self.nbytes, _s(stream)))
 
     def free(self):
         if self.ptr and self._base is None:
@@ -312,6 +313,10 @@ class Stream:
 
     def sync(self):
         check(lib().tdbp_stream_sync(self.ptr))
+
+    def wait(self, event: "Event"):
+        """Later work on this stream waits for `event`."""
+        check(lib().tdbp_stream_wait_event(self.ptr, event.ptr))
 
     def __del__(self):
         try:
