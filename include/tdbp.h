@@ -261,10 +261,11 @@ int tdbp_backproject_variant(const tdbp_geom *g, const tdbp_samples *s,
                              const double *d_coef, void *d_out, int precision,
                              int variant, void *stream);
 
-/* Autotuner: time every kernel variant on this burst (reps launches each,
- * after one warm-up), return the fastest in *best and, for TDBP_MIXED, make
- * it the default of later launches.  ms (optional, [count]) gets the mean
- * milliseconds per launch of each variant.  Overwrites d_out. */
+/* Autotuner: time every VALU kernel variant on this burst (reps launches
+ * each, after one warm-up), return the fastest in *best and, for TDBP_MIXED,
+ * make it the default of later launches.  ms (optional, [count]) gets the
+ * mean milliseconds per launch of each variant (0 for the MFMA variant 8,
+ * which runs only when asked for by index).  Overwrites d_out. */
 int tdbp_autotune(const tdbp_geom *g, const tdbp_samples *s,
                   const double *d_satloc, const int16_t *d_dem, void *d_out,
                   int precision, int reps, float *ms, int *best);
