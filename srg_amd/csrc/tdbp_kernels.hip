@@ -220,9 +220,12 @@ __device__ __forceinline__ void plain_tile(const BurstDev& b, int tile) {
 //   r0 = (du.x, du.y, du.z, c0)   du = u_p - u_ref, c0 = frac(2 R0/lambda)
 //   r1 = (f0, i0*elem bits, 1/(2 R0), scale)  (R0 - rngstart)/dmrg = i0 + f0
 //   r2 = (safe, lim, i0 bits, -)   lim: rho <= lim <=> r <= r_hi; safe != 0:
-//        every pixel of the workgroup tile is inside the range gate
+//        every pixel of the workgroup tile is inside the range gate (and, for
+//        the quadratic form, its local index is at least 0)
+//   r3 = (k, f0s, c0, scale)       k = q - pc, pc the central pulse of the tile
+//        window; (R0 - rngstart)/dmrg = i0s + f0s with i0s fixed per stage
 struct PulseRec {
-  float4 r0, r1, r2;
+  float4 r0, r1, r2, r3;
 };
 
 constexpr int kStage = kBlock;  // pulses per stage = one per thread
@@ -232,6 +235,7 @@ struct TileShared {
   double x0[3];      // reference point
   int key, wlo, whi;
   unsigned dmax;     // bits of max |x - x0| over live pixels (float, >= 0)
+  unsigned pmax;     // bits of max |sd| of the quadratics over their windows
 };
 
 // Global-address-space view of the sample rows: keeps the gather a
@@ -243,6 +247,31 @@ typedef float F32x4 __attribute__((ext_vector_type(4), aligned(8)));
 template <class T>
 __device__ __forceinline__ T gload(grow_t p) {
   return *(const __attribute__((address_space(1))) T*)p;
+}
+
+// Structured-buffer view of a burst's samples (stride = one sample), so a
+// gather addresses sample vindex at voffset + soffset bytes with no address
+// arithmetic on the vector unit.  The base sits kSbufBias bytes below the
+// samples so the per-pulse scalar offset stays non-negative; num_records is
+// the maximum (the callers keep every index inside the row).
+typedef int I32x4 __attribute__((ext_vector_type(4)));
+typedef float SbufF2 __attribute__((ext_vector_type(2)));
+typedef float SbufF4 __attribute__((ext_vector_type(4)));
+__device__ SbufF2 sbuf_load_v2f32(I32x4 rsrc, int vindex, int voffset, int soffset, int aux)
+    __asm("llvm.amdgcn.struct.buffer.load.v2f32");
+__device__ SbufF4 sbuf_load_v4f32(I32x4 rsrc, int vindex, int voffset, int soffset, int aux)
+    __asm("llvm.amdgcn.struct.buffer.load.v4f32");
+constexpr long kSbufBias = 1l << 20;
+
+__device__ __forceinline__ I32x4 sample_rsrc(const void* data, int elem) {
+  const unsigned long long a = (unsigned long long)data - (unsigned long long)kSbufBias;
+  I32x4 r;
+  r.x = __builtin_amdgcn_readfirstlane((int)(unsigned)a);
+  r.y = __builtin_amdgcn_readfirstlane((int)(((unsigned)(a >> 32) & 0xffffu) |
+                                             ((unsigned)elem << 16)));
+  r.z = -1;                 // num_records: no index limit
+  r.w = 0x00020000;         // 32-bit data format
+  return r;
 }
 
 // Sample pair (i, i+1) at byte offset `off` from a uniform row base, lerped
@@ -319,14 +348,60 @@ struct MixedPixel {
   float w;
 };
 
+// The range excess of a pixel over its pulse window as a quadratic in the
+// pulse index k = q - pc (pc: the tile window's central pulse), fitted per
+// pixel in FP64 (mixed_tile).  Over one wave's window the excess is smooth
+// to far below the quadratic's remainder: on IW geometry the fit is good to
+// 2e-8 m, the FP32 Horner evaluation to 3e-8 m (0.006 mrad of phase).
+//
+// The quadratic also carries the pixel's sub-sample range offset: with
+// aref/dmrg = ia + fa (ia = rint), P(k) = sd + fa dmrg, so the local range
+// index is rho = P/dmrg + f0s (one FMA) and the sample index i0s + ia +
+// floor(rho).  The row pointer of a pulse carries (i0s + ia_min) * elem,
+// the lane (ia - ia_min) * elem; the phase fa dmrg 2/lambda that P adds is
+// taken out once per pixel at the end.
+struct PolyPixel {
+  float c0, c1, c2;
+  unsigned iaoff;    // (ia - ia_min) * elem
+};
+
+// One pair on the quadratic.  rows: row start shifted by (i0s + ia_min) *
+// elem (unchecked pairs); rowq: the row start (checked pairs, which address
+// by the sample index i0s + ia + floor(rho) = ibase + floor(rho)).  lim:
+// rho <= lim <=> r <= r_hi.
+template <int DT, bool CHECK>
+__device__ __forceinline__ void poly_pair(const PolyPixel& pp, const float4& r3, grow_t rowq,
+                                          grow_t rows, float Kf, float invd, bool inwin,
+                                          int ibase, unsigned smax, float lim, float& sr,
+                                          float& si) {
+  const float P = fmaf(fmaf(pp.c2, r3.x, pp.c1), r3.x, pp.c0);
+  const float rho = fmaf(P, invd, r3.y);
+  int il;
+  asm("v_cvt_flr_i32_f32 %0, %1" : "=v"(il) : "v"(rho));
+  float2 c;
+  const float fr = __builtin_amdgcn_fractf(rho);
+  if constexpr (CHECK) {
+    const int idx = il + ibase;
+    if (!(inwin && (unsigned)idx <= smax && rho <= lim)) return;
+    c = lerp_pair<DT>(rowq, (unsigned)idx * (unsigned)elem_bytes<DT>(), fr, r3.w);
+  } else {
+    c = lerp_pair<DT>(rows, (unsigned)il * (unsigned)elem_bytes<DT>() + pp.iaoff, fr, r3.w);
+  }
+  const float cyc = fmaf(Kf, P, r3.z);
+  const float sn = __builtin_amdgcn_sinf(cyc), cs = __builtin_amdgcn_cosf(cyc);
+  sr = fmaf(c.x, cs, sr);
+  sr = fmaf(-c.y, sn, sr);
+  si = fmaf(c.x, sn, si);
+  si = fmaf(c.y, cs, si);
+}
+
 // One pixel-pulse pair; CHECK selects the per-lane window and gate tests.
 // rowb is the uniform start of pulse row q; the lane's byte offset is
 // (floor(rho) + i0) * elem, non-negative whenever the gate holds.
 template <int DT, bool CHECK>
-__device__ __forceinline__ void mixed_pair(const MixedPixel& m, const PulseRec& r,
-                                           grow_t rowb, float Kf, float invd,
-                                           bool inwin, unsigned smax, float& sr,
-                                           float& si) {
+__device__ __forceinline__ void mixed_pair(const MixedPixel& m, const PulseRec& r, grow_t rowb,
+                                           float Kf, float invd, bool inwin, unsigned smax,
+                                           float& sr, float& si) {
   const float da = fmaf(r.r0.x, m.dx, fmaf(r.r0.y, m.dy, r.r0.z * m.dz));
   const float e = fmaf(m.w, da, m.t1);
   const float t = fmaf(-e, da, m.t0);             // w (bb - a^2)
@@ -353,29 +428,49 @@ __device__ __forceinline__ void mixed_pair(const MixedPixel& m, const PulseRec& 
 // The test-free pair split in two for a one-pulse software pipeline: the
 // front computes the range, issues the sample load and the phase argument;
 // the back (a pulse later) lerps and accumulates.  Same arithmetic as
-// mixed_pair<DT, false>.
+// mixed_pair<DT, false> (POLY false) or poly_pair<DT, false> (POLY true).
 template <int DT>
 struct PairFront {
   typename RawPair<DT>::type v;
   float fr, cyc, sc;
 };
-template <int DT>
-__device__ __forceinline__ PairFront<DT> mixed_front(const MixedPixel& m, const PulseRec& r,
-                                                     grow_t rowb, float Kf, float invd) {
-  const float da = fmaf(r.r0.x, m.dx, fmaf(r.r0.y, m.dy, r.r0.z * m.dz));
-  const float e = fmaf(m.w, da, m.t1);
-  const float t = fmaf(-e, da, m.t0);
-  const float sd = fmaf(t, r.r1.z, da);
-  const float rho = fmaf(sd, invd, r.r1.x + m.arefd);
-  int il;
-  asm("v_cvt_flr_i32_f32 %0, %1" : "=v"(il) : "v"(rho));
-  const unsigned off =
-      (unsigned)il * (unsigned)elem_bytes<DT>() + (unsigned)__float_as_int(r.r1.y);
+// POLY: the sample pair comes through the structured buffer `rsrc` at
+// scalar offset `soff` (row start + (i0s + ia_min) * elem + kSbufBias).
+template <int DT, bool POLY>
+__device__ __forceinline__ PairFront<DT> mixed_front(const MixedPixel& m, const PolyPixel& pp,
+                                                     const PulseRec& r, grow_t rowb,
+                                                     const I32x4& rsrc, unsigned soff, float Kf,
+                                                     float invd) {
   PairFront<DT> f;
-  f.v = gload<typename RawPair<DT>::type>(rowb + off);
-  f.fr = __builtin_amdgcn_fractf(rho);
-  f.cyc = fmaf(Kf, sd, r.r0.w);
-  f.sc = r.r1.w;
+  if constexpr (POLY) {
+    const float P = fmaf(fmaf(pp.c2, r.r3.x, pp.c1), r.r3.x, pp.c0);
+    const float rho = fmaf(P, invd, r.r3.y);
+    int il;
+    asm("v_cvt_flr_i32_f32 %0, %1" : "=v"(il) : "v"(rho));
+    // Measured on the 27-burst IW scene: 1.5% above the same loop with a
+    // global_load (whose lane offset costs one more VALU op per pair).
+    if constexpr (DT == TDBP_CF32)
+      f.v = __builtin_bit_cast(F32x4, sbuf_load_v4f32(rsrc, il, (int)pp.iaoff, (int)soff, 0));
+    else
+      f.v = __builtin_bit_cast(U32x2, sbuf_load_v2f32(rsrc, il, (int)pp.iaoff, (int)soff, 0));
+    f.fr = __builtin_amdgcn_fractf(rho);
+    f.cyc = fmaf(Kf, P, r.r3.z);
+    f.sc = r.r3.w;
+  } else {
+    const float da = fmaf(r.r0.x, m.dx, fmaf(r.r0.y, m.dy, r.r0.z * m.dz));
+    const float e = fmaf(m.w, da, m.t1);
+    const float t = fmaf(-e, da, m.t0);
+    const float sd = fmaf(t, r.r1.z, da);
+    const float rho = fmaf(sd, invd, r.r1.x + m.arefd);
+    int il;
+    asm("v_cvt_flr_i32_f32 %0, %1" : "=v"(il) : "v"(rho));
+    const unsigned off =
+        (unsigned)il * (unsigned)elem_bytes<DT>() + (unsigned)__float_as_int(r.r1.y);
+    f.v = gload<typename RawPair<DT>::type>(rowb + off);
+    f.fr = __builtin_amdgcn_fractf(rho);
+    f.cyc = fmaf(Kf, sd, r.r0.w);
+    f.sc = r.r1.w;
+  }
   return f;
 }
 template <int DT>
@@ -388,6 +483,113 @@ __device__ __forceinline__ void mixed_back(const PairFront<DT>& f, float& sr, fl
   si = fmaf(c.y, cs, si);
 }
 
+// One stage's pulses [qa, qb) for one wave: checked edges [qa, ea) and
+// [eb, qb), the core [ca, cb) where every live lane is inside its window.
+// stage_safe: every pulse of the stage keeps the whole tile inside the gate.
+// POLY: the quadratic (shift = (i0s + ia_min) * elem, ibase = i0s + ia, lim
+// = r_hi - i0s - ia); else the FP32 chain.
+template <int DT, bool POLY>
+__device__ __forceinline__ void stage_pulses(const MixedPixel& m, const PolyPixel& pp,
+                                             const PulseRec* rec, int p0, int qa, int qb,
+                                             int qc0, int qc1, bool stage_safe, grow_t base,
+                                             const I32x4& rsrc, size_t rowbytes, long shift,
+                                             int ibase, float lim,
+                                             float Kf, float invd, int azs, unsigned n,
+                                             unsigned smax, float& sr, float& si) {
+  const int ea = min(qb, qc0);                      // [qa, ea): checked
+  const int ca = max(qa, qc0), cb = min(qb, qc1);   // core
+  const int eb = max(qa, qc1);                      // [eb, qb): checked
+  auto checked = [&](int qq) {
+    const grow_t rowq = base + (size_t)qq * rowbytes;
+    if constexpr (POLY)
+      poly_pair<DT, true>(pp, rec[qq - p0].r3, rowq, rowq, Kf, invd, (unsigned)(qq - azs) < n,
+                          ibase, smax, lim, sr, si);
+    else
+      mixed_pair<DT, true>(m, rec[qq - p0], rowq, Kf, invd, (unsigned)(qq - azs) < n, smax, sr,
+                           si);
+  };
+  auto unchecked = [&](int qq) {
+    const grow_t rowq = base + (size_t)qq * rowbytes;
+    if constexpr (POLY)
+      poly_pair<DT, false>(pp, rec[qq - p0].r3, rowq, rowq + shift, Kf, invd, true, ibase,
+                           smax, lim, sr, si);
+    else
+      mixed_pair<DT, false>(m, rec[qq - p0], rowq, Kf, invd, true, smax, sr, si);
+  };
+  for (int qq = qa; qq < ea; ++qq) checked(qq);
+  if (stage_safe) {
+    // No tests.  Two pairs in flight: each sample load is issued a whole pair
+    // ahead of its use (fa and fb keep their own registers; the scheduling
+    // barrier after each front keeps the load there).  An odd count starts
+    // with one plain pair; an empty core (cb <= ca) does nothing.  Measured
+    // with the FP32 chain: one pair in flight 1240, two 1442-1546, three 1%
+    // below two Gpixel-pulse/s on the 27-burst IW scene.
+    int qq = ca;
+    if (qq < cb && ((cb - qq) & 1)) {
+      unchecked(qq);
+      ++qq;
+    }
+    if (qq < cb) {
+      grow_t rp = base + (size_t)qq * rowbytes;
+      unsigned so = (unsigned)((long)qq * (long)rowbytes + shift + kSbufBias);
+      const unsigned rb32 = (unsigned)rowbytes;
+      const PulseRec* rr = rec + (qq - p0);
+      PairFront<DT> fa = mixed_front<DT, POLY>(m, pp, rr[0], rp, rsrc, so, Kf, invd);
+      PairFront<DT> fb;
+      for (; qq + 2 < cb; qq += 2) {
+        fb = mixed_front<DT, POLY>(m, pp, rr[1], rp + rowbytes, rsrc, so + rb32, Kf, invd);
+        __builtin_amdgcn_sched_barrier(0);
+        mixed_back<DT>(fa, sr, si);
+        rp += 2 * rowbytes;
+        so += 2 * rb32;
+        rr += 2;
+        fa = mixed_front<DT, POLY>(m, pp, rr[0], rp, rsrc, so, Kf, invd);
+        __builtin_amdgcn_sched_barrier(0);
+        mixed_back<DT>(fb, sr, si);
+      }
+      fb = mixed_front<DT, POLY>(m, pp, rr[1], rp + rowbytes, rsrc, so + rb32, Kf, invd);
+      mixed_back<DT>(fa, sr, si);
+      mixed_back<DT>(fb, sr, si);
+    }
+  } else {
+    for (int qq = ca; qq < cb; ++qq) {
+      if (__builtin_amdgcn_readfirstlane(__float_as_int(rec[qq - p0].r2.x)) != 0)
+        unchecked(qq);
+      else
+        checked(qq);
+    }
+  }
+  for (int qq = eb; qq < qb; ++qq) checked(qq);
+}
+
+// 1/sqrt(x) in FP64 for normal x > 0: the hardware estimate and two Newton
+// steps y <- y (3/2 - x/2 y^2), each squaring the relative error.
+__device__ __forceinline__ double rsqrt64(double x) {
+  double y = __builtin_amdgcn_rsq(x);
+  const double h = 0.5 * x;
+  y = y * fma(-h * y, y, 1.5);
+  y = y * fma(-h * y, y, 1.5);
+  return y;
+}
+
+// Lane `l`'s double, moved to scalar registers.
+__device__ __forceinline__ double lane_double(double v, int l) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffll), l);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
+// Range excess R - |x0 - s_q| - aref of pixel x0 + d at pulse q, in FP64,
+// from the pulse's look vector u = (x0 - s_q)/R0 and h = 1/(2 R0): with a =
+// u.d and |d|^2 = bb, R - R0 - a = (bb - a^2)/(R + R0 + a) and 1/(R + R0 +
+// a) = h (1 - q + q^2) with q = 2 h a, to 1e-10 m at satellite ranges.
+__device__ __forceinline__ double node_excess(double u0, double u1, double u2, double h,
+                                              const double* d, double bb, double aref) {
+  const double a = u0 * d[0] + u1 * d[1] + u2 * d[2];
+  const double q = 2.0 * h * a;
+  return (a - aref) + (bb - a * a) * fma(q, q - 1.0, 1.0) * h;
+}
 
 template <int TR, bool STACK, int DT>
 __device__ __forceinline__ void mixed_tile(const BurstDev& b, int tile,
@@ -411,6 +613,7 @@ __device__ __forceinline__ void mixed_tile(const BurstDev& b, int tile,
     sh->wlo = 0x7fffffff;
     sh->whi = -1;
     sh->dmax = 0u;
+    sh->pmax = 0u;
   }
   __syncthreads();
   if (live) {
@@ -451,18 +654,6 @@ __device__ __forceinline__ void mixed_tile(const BurstDev& b, int tile,
                          live ? x[2] - x0[2] : 0.};
   const double bb64 = d64[0] * d64[0] + d64[1] * d64[1] + d64[2] * d64[2];
   const double aref64 = uref[0] * d64[0] + uref[1] * d64[1] + uref[2] * d64[2];
-  const double q = aref64 / R0ref;
-  const double w64 = 1.0 - q + q * q;
-  MixedPixel m;
-  m.dx = (float)d64[0];
-  m.dy = (float)d64[1];
-  m.dz = (float)d64[2];
-  m.t0 = (float)(w64 * (bb64 - aref64 * aref64));
-  m.t1 = (float)(2.0 * w64 * aref64);
-  m.arefd = (float)(aref64 * b.inv_dmrg);
-  m.w = (float)w64;
-  const double cp64 = aref64 * b.two_over_wvl;
-  const float cpix = (float)(cp64 - rint(cp64));
   if (live) atomicMax(&sh->dmax, __float_as_uint((float)sqrt(bb64)));
   const float Kf = (float)b.two_over_wvl;
   const float invd = (float)b.inv_dmrg;
@@ -477,13 +668,106 @@ __device__ __forceinline__ void mixed_tile(const BurstDev& b, int tile,
   const bool wave_live = qhi > qlo;
   const unsigned n = (unsigned)(p.aze - p.azs);
   const grow_t base = (grow_t)b.data;
+  const I32x4 rsrc = sample_rsrc(b.data, elem_bytes<DT>());
   const size_t rowbytes = (size_t)b.S * elem_bytes<DT>();
   const unsigned smax = (unsigned)(b.S - 2);
+
+  // The range excess as a quadratic in k = q - pc: interpolated at three
+  // Chebyshev nodes of the wave window, checked against the exact excess at
+  // the window's two ends.  A wave where a live pixel misses by more than
+  // 1e-7 m (0.023 mrad of phase) keeps the FP32 chain.  Lanes without a
+  // pixel sit on x0 (excess 0).
+  const double arefd64 = aref64 * b.inv_dmrg;
+  const int ia = (int)rint(arefd64);
+  const double fad = aref64 - ia * b.dmrg;          // fa dmrg, |fa| <= 1/2
+  const int ia_min = wave_min(ia);
+  PolyPixel pp = {0.f, 0.f, 0.f, (unsigned)(ia - ia_min) * (unsigned)elem_bytes<DT>()};
+  bool wave_poly = false;
+  if (wave_live && qhi - qlo >= 16) {
+    const double mid = 0.5 * (qlo + qhi - 1), hh = 0.5 * (qhi - 1 - qlo);
+    const int n0 = (int)rint(mid - 0.8660254037844386 * hh);
+    const int n1 = (int)rint(mid);
+    const int n2 = (int)rint(mid + 0.8660254037844386 * hh);
+    // Lane j < 5 forms the look vector of fit pulse j (nodes, then the
+    // window's ends); every lane reads them back.
+    const int lane = tid & 63;
+    double gu0 = 0., gu1 = 0., gu2 = 0., gh = 0.;
+    if (lane < 5) {
+      const int qn = lane == 0 ? n0 : lane == 1 ? n1 : lane == 2 ? n2 : lane == 3 ? qlo : qhi - 1;
+      const double* sq = b.satloc + 3 * qn;
+      const double v0 = x0[0] - sq[0], v1 = x0[1] - sq[1], v2 = x0[2] - sq[2];
+      const double inv = rsqrt64(v0 * v0 + v1 * v1 + v2 * v2);
+      gu0 = v0 * inv;
+      gu1 = v1 * inv;
+      gu2 = v2 * inv;
+      gh = 0.5 * inv;
+    }
+    double y[5];
+#pragma unroll
+    for (int j = 0; j < 5; ++j)
+      y[j] = node_excess(lane_double(gu0, j), lane_double(gu1, j), lane_double(gu2, j),
+                         lane_double(gh, j), d64, bb64, aref64);
+    const double k0 = n0 - pc, k1 = n1 - pc, k2 = n2 - pc;
+    const double a1 = (y[1] - y[0]) / (k1 - k0);
+    const double a2 = ((y[2] - y[1]) / (k2 - k1) - a1) / (k2 - k0);
+    const double c2 = a2, c1 = a1 - a2 * (k0 + k1), c0 = y[0] - k0 * (a1 - a2 * k1);
+    const double kl = qlo - pc, kh = qhi - 1 - pc;
+    const double el = fabs((c2 * kl + c1) * kl + c0 - y[3]);
+    const double eh = fabs((c2 * kh + c1) * kh + c0 - y[4]);
+    // |R_q(x) - R_q(x0)| <= |d| for any pulse position, so |sd| <= 2 |d|;
+    // the quadratic must keep that bound over the whole window (ends and
+    // vertex), which also bounds every index it can produce.
+    double pm = fmax(fabs((c2 * kl + c1) * kl + c0),
+                     fabs((c2 * kh + c1) * kh + c0));
+    const double kv = c2 != 0.0 ? -c1 / (2.0 * c2) : kl;
+    if (kv > kl && kv < kh) pm = fmax(pm, fabs((c2 * kv + c1) * kv + c0));
+    const bool ok = el <= 1e-7 && eh <= 1e-7 &&   // false for NaN
+                    pm <= 2.0 * sqrt(bb64) + 1e-3 &&
+                    (size_t)b.L * rowbytes + (size_t)kSbufBias < (size_t)0x7fffffff;
+    wave_poly = wave_min(ok ? 1 : 0) == 1;
+    if (wave_poly && live) atomicMax(&sh->pmax, __float_as_uint((float)(pm * (1.0 + 1e-6))));
+    pp.c0 = (float)(c0 + fad);
+    pp.c1 = (float)c1;
+    pp.c2 = (float)c2;
+  }
+  // Per-pixel constants of the FP32 chain (waves without the quadratic).
+  MixedPixel m = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+  if (!wave_poly) {
+    const double q = aref64 / R0ref;
+    const double w64 = 1.0 - q + q * q;
+    m.dx = (float)d64[0];
+    m.dy = (float)d64[1];
+    m.dz = (float)d64[2];
+    m.t0 = (float)(w64 * (bb64 - aref64 * aref64));
+    m.t1 = (float)(2.0 * w64 * aref64);
+    m.arefd = (float)(aref64 * b.inv_dmrg);
+    m.w = (float)w64;
+  }
+  // Phase taken out at the end: frac(2/lambda aref), less Kf fa dmrg on the
+  // quadratic, whose cycles Kf P carry it.
+  const double cp64 = aref64 * b.two_over_wvl - (wave_poly ? (double)Kf * fad : 0.0);
+  const float cpix = (float)(cp64 - rint(cp64));
   __syncthreads();
-  // Conservative range extent of the tile about x0: R - R0 in
-  // [-dmax, dmax + dmax^2/(2(R0-dmax))], plus one sample of margin.
+  // max |x - x0| over the live pixels of the tile.
   const double dmax =
       (double)__uint_as_float((unsigned)__builtin_amdgcn_readfirstlane((int)sh->dmax));
+  // Bound on |P| of every quadratic in use: |sd| <= pmx, |fa dmrg| <= dmrg/2.
+  const double pmx =
+      (double)__uint_as_float((unsigned)__builtin_amdgcn_readfirstlane((int)sh->pmax)) + 1e-6;
+
+  // Sample base of the quadratic form for the tile: 64 + pmx/dmrg samples
+  // below the range of x0 at the central pulse, so f0s = rb - i0s stays >= 1
+  // + pmx/dmrg + 1/2 unless the range walks by more than 62 samples over the
+  // window (each pulse checks it below).
+  const int i0s = __builtin_amdgcn_readfirstlane(
+      (int)floor((R0ref - b.rngstart) * b.inv_dmrg) - 64 - (int)fmin(pmx * b.inv_dmrg, 4096.0));
+  // Range extent of the tile about x0 in samples: R - R0 in [-dmax, dmax +
+  // dmax^2/(2(R0 - dmax))], the latter <= dmax^2/R0ref for R0 >= R0ref/2 +
+  // dmax (checked per pulse); one sample of margin each side.
+  const double ext_lo = dmax * b.inv_dmrg + 1.0;
+  const double ext_hi = (dmax + dmax * dmax / R0ref) * b.inv_dmrg + 1.0;
+  const double r0min = 0.5 * R0ref + dmax;
+  const double mq = (dmax + pmx + 1e-3) * b.inv_dmrg;
 
   float acc_r = 0.f, acc_i = 0.f, cmp_r = 0.f, cmp_i = 0.f;
   for (int p0 = wlo; p0 < whi; p0 += kStage) {
@@ -493,22 +777,31 @@ __device__ __forceinline__ void mixed_tile(const BurstDev& b, int tile,
     if (qs < whi) {
       const double* s = b.satloc + 3 * qs;
       const double u0 = x0[0] - s[0], u1 = x0[1] - s[1], u2 = x0[2] - s[2];
-      const double R0 = sqrt(u0 * u0 + u1 * u1 + u2 * u2);
-      const double inv = 1.0 / R0;
+      const double r2 = u0 * u0 + u1 * u1 + u2 * u2;
+      const double inv = rsqrt64(r2);
+      const double R0 = r2 * inv;
       const double c = R0 * b.two_over_wvl;
       const double rb = (R0 - b.rngstart) * b.inv_dmrg;
       const double i0 = floor(rb);
-      const double rlo = rb - dmax * b.inv_dmrg - 1.0;
-      const double rhi = rb + (dmax + dmax * dmax / (2.0 * (R0 - dmax))) * b.inv_dmrg + 1.0;
-      const bool safe = rlo >= 1.0 && rhi <= b.r_hi - 1.0;
+      const double rlo = rb - ext_lo;
+      const double rhi = rb + ext_hi;
+      const double du0 = u0 * inv - uref[0], du1 = u1 * inv - uref[1],
+                   du2 = u2 * inv - uref[2];
+      // Quadratic form: |P| <= pmx + dmrg/2 and |ia| <= dmax/dmrg + 1/2, so
+      // the local index P/dmrg + f0s is >= 1 and the sample index i0s + ia +
+      // floor(rho) (and its neighbour) lies in [0, S - 1] when these hold.
+      const double f0s = rb - i0s;
+      const bool safe = R0 >= r0min && rlo >= 1.0 && rhi <= b.r_hi - 1.0 &&
+                        f0s - pmx * b.inv_dmrg - 1.5 >= 0.0 && f0s <= 1024.0 &&
+                        rb - mq - 3.0 >= 0.0 && rb + mq + 3.0 <= b.S - 1;
       PulseRec r;
-      r.r0 = make_float4((float)(u0 * inv - uref[0]), (float)(u1 * inv - uref[1]),
-                         (float)(u2 * inv - uref[2]), (float)(c - rint(c)));
+      const float cf = (float)(c - rint(c));
+      r.r0 = make_float4((float)du0, (float)du1, (float)du2, cf);
       const float sc = (DT == TDBP_CF32 || b.scale == nullptr) ? 1.f : b.scale[qs];
       r.r1 = make_float4((float)(rb - i0), __int_as_float((int)i0 * elem_bytes<DT>()),
                          (float)(0.5 * inv), sc);
-      r.r2 = make_float4(safe ? 1.f : 0.f, (float)(b.r_hi - i0), __int_as_float((int)i0),
-                         0.f);
+      r.r2 = make_float4(safe ? 1.f : 0.f, (float)(b.r_hi - i0), __int_as_float((int)i0), 0.f);
+      r.r3 = make_float4((float)(qs - pc), (float)f0s, cf, sc);
       rec[tid] = r;
       safe_all = safe;
     }
@@ -519,60 +812,13 @@ __device__ __forceinline__ void mixed_tile(const BurstDev& b, int tile,
     // are sentinels and must not enter index arithmetic).
     if (wave_live) {
       const int qa = max(p0, qlo), qb = min(p0 + kStage, qhi);
-      const int ea = min(qb, qc0);                      // [qa, ea): checked
-      const int ca = max(qa, qc0), cb = min(qb, qc1);   // core
-      const int eb = max(qa, qc1);                      // [eb, qb): checked
-      for (int qq = qa; qq < ea; ++qq)
-        mixed_pair<DT, true>(m, rec[qq - p0], base + (size_t)qq * rowbytes, Kf, invd,
-                             (unsigned)(qq - p.azs) < n, smax, sr, si);
-      if (stage_safe) {
-        // Every pulse of the stage keeps the whole tile inside the gate and
-        // every live lane of the wave is inside its window: no tests.
-        int qq = ca;
-        // Two pairs in flight: each sample load is issued a whole pair ahead
-        // of its use (fa and fb keep their own registers; the scheduling
-        // barrier after each front keeps the load there).  An odd count
-        // starts with one plain pair; an empty core (cb <= ca) does nothing.
-        // Measured: one pair in flight 1240, two 1442-1546, three 1% below
-        // two Gpixel-pulse/s on the 27-burst IW scene.
-        if (qq < cb && ((cb - qq) & 1)) {
-          mixed_pair<DT, false>(m, rec[qq - p0], base + (size_t)qq * rowbytes, Kf, invd, true,
-                                smax, sr, si);
-          ++qq;
-        }
-        if (qq < cb) {
-          grow_t rp = base + (size_t)qq * rowbytes;
-          const PulseRec* rr = rec + (qq - p0);
-          PairFront<DT> fa = mixed_front<DT>(m, rr[0], rp, Kf, invd);
-          PairFront<DT> fb;
-          for (; qq + 2 < cb; qq += 2) {
-            fb = mixed_front<DT>(m, rr[1], rp + rowbytes, Kf, invd);
-            __builtin_amdgcn_sched_barrier(0);
-            mixed_back<DT>(fa, sr, si);
-            rp += 2 * rowbytes;
-            rr += 2;
-            fa = mixed_front<DT>(m, rr[0], rp, Kf, invd);
-            __builtin_amdgcn_sched_barrier(0);
-            mixed_back<DT>(fb, sr, si);
-          }
-          fb = mixed_front<DT>(m, rr[1], rp + rowbytes, Kf, invd);
-          mixed_back<DT>(fa, sr, si);
-          mixed_back<DT>(fb, sr, si);
-        }
-      } else {
-        for (int qq = ca; qq < cb; ++qq) {
-          const PulseRec r = rec[qq - p0];
-          const grow_t rowb = base + (size_t)qq * rowbytes;
-          if (__builtin_amdgcn_readfirstlane(__float_as_int(r.r2.x)) != 0)
-            mixed_pair<DT, false>(m, r, rowb, Kf, invd, true, smax, sr, si);
-          else
-            mixed_pair<DT, true>(m, r, rowb, Kf, invd, (unsigned)(qq - p.azs) < n, smax,
-                                 sr, si);
-        }
-      }
-      for (int qq = eb; qq < qb; ++qq)
-        mixed_pair<DT, true>(m, rec[qq - p0], base + (size_t)qq * rowbytes, Kf, invd,
-                             (unsigned)(qq - p.azs) < n, smax, sr, si);
+      if (wave_poly)
+        stage_pulses<DT, true>(m, pp, rec, p0, qa, qb, qc0, qc1, stage_safe != 0, base, rsrc,
+                               rowbytes, (long)(i0s + ia_min) * elem_bytes<DT>(), i0s + ia,
+                               (float)(b.r_hi - i0s - ia), Kf, invd, p.azs, n, smax, sr, si);
+      else
+        stage_pulses<DT, false>(m, pp, rec, p0, qa, qb, qc0, qc1, stage_safe != 0, base, rsrc,
+                                rowbytes, 0, 0, 0.f, Kf, invd, p.azs, n, smax, sr, si);
     }
     // Kahan-compensated join of the stage sums.
     {
