@@ -141,6 +141,10 @@ void fill_burst(const tdbp_geom* g, BurstDev* b) {
   b->inv_dmrg = 1.0 / g->dmrg;
   b->two_over_wvl = 2.0 / g->wvl;
   b->cpu = g->semantics == TDBP_SEM_CPU;
+  {
+    const char* ec = std::getenv("TDBP_MIXED_CHAIN");
+    b->chain = ec != nullptr && std::atoi(ec) != 0;
+  }
   b->dmrg = g->dmrg;
   b->four_pi_over_wvl = 4. * kPi / g->wvl;  // sentinel/azimuth_compress_cpu.c:75
   b->r_hi = std::min((g->rngend - g->dmrg - g->rngstart) / g->dmrg,

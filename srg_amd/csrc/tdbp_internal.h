@@ -32,6 +32,8 @@ struct BurstDev {
   int dtype, nsv;
   int tiles_r, tiles_c;    // tile grid of this burst for the chosen variant
   int cpu;                 // 1: CPU-twin arithmetic (TDBP_SEM_CPU)
+  int chain;               // 1: the mixed kernel keeps the FP32 range chain
+                           //    (TDBP_MIXED_CHAIN=1; default: the quadratic form)
   double rngstart, rng_hi, inv_dmrg, two_over_wvl, r_hi;
   double dmrg, four_pi_over_wvl;  // CPU twin: (R - rngstart)/dmrg, 4 pi/wvl * R
   double firstlat, firstlon, deltalat, deltalon;

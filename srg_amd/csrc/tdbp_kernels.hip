@@ -683,7 +683,7 @@ __device__ __forceinline__ void mixed_tile(const BurstDev& b, int tile,
   const int ia_min = wave_min(ia);
   PolyPixel pp = {0.f, 0.f, 0.f, (unsigned)(ia - ia_min) * (unsigned)elem_bytes<DT>()};
   bool wave_poly = false;
-  if (wave_live && qhi - qlo >= 16) {
+  if (wave_live && qhi - qlo >= 16 && !b.chain) {
     const double mid = 0.5 * (qlo + qhi - 1), hh = 0.5 * (qhi - 1 - qlo);
     const int n0 = (int)rint(mid - 0.8660254037844386 * hh);
     const int n1 = (int)rint(mid);

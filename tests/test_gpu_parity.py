@@ -141,19 +141,75 @@ def test_plain_fp64_matches_oracle(scene, request):
 BOUNDS = {T.CF32: (1e-3, 1e-4), T.CF16: (1e-3, 1e-4), T.CBF16: (5e-3, 5e-3)}
 
 
+@pytest.mark.parametrize("form", ["quadratic", "chain"])
 @pytest.mark.parametrize("scene", ["c1", "fbm_small"])
 @pytest.mark.parametrize("dtype", [T.CF32, T.CF16, T.CBF16])
-def test_mixed_matches_oracle(scene, dtype, request):
+def test_mixed_matches_oracle(scene, dtype, form, request, monkeypatch):
+    """Both range forms of the mixed kernel: the per-pixel quadratic in the
+    pulse index (default) and the FP32 chain it falls back to
+    (TDBP_MIXED_CHAIN=1)."""
+    if form == "chain":
+        monkeypatch.setenv("TDBP_MIXED_CHAIN", "1")
     d, ref = request.getfixturevalue(scene)
     d.fit()
     got = d.run(T.MIXED, dtype)
     m = ref["azoff"] > 0
     ph, amp = errors(got, ref["out"], m)
-    print(f"{scene} mixed {T.DTYPE_NAMES[dtype]} vs oracle: phase {ph*1e3:.4f} mrad, "
+    print(f"{scene} mixed {form} {T.DTYPE_NAMES[dtype]} vs oracle: phase {ph*1e3:.4f} mrad, "
           f"amp {amp:.3e}")
     pb, ab = BOUNDS[dtype]
     assert ph <= pb and amp <= ab
     assert np.all(got[~m] == 0)
+
+
+@pytest.fixture(scope="module")
+def jittered(gpu, oracle):
+    """A 512-pulse burst on a coarse 0.0006 deg column posting (16 x 16 pixel
+    tiles ~0.9 km across) with every pulse position moved by N(0, 2 mm) per
+    axis after the samples were made: pulse positions off any smooth path."""
+    sc = synth.make_burst(512, 4096, 128, 96, dlat=-0.0002, dlon=0.0006, aperture=256,
+                          dem="fbm", dem_source=synth.fbm_surface((256, 256), seed=5),
+                          ntargets=3, clutter_sigma=0.1, seed=5)
+    d = Dev(sc)
+    sat = sc.satloc + np.random.default_rng(5).normal(0.0, 0.002, sc.satloc.shape)
+    d.d_sat = T.DeviceArray.from_host(sat)
+    out, acc, pairs, coef, azoff = oracle.backproject(sc.geom, d.data_h, sat, sc.dem)
+    return d, dict(out=out, acc=acc, pairs=pairs, coef=coef, azoff=azoff)
+
+
+@pytest.mark.parametrize("form", ["quadratic", "chain"])
+def test_mixed_irregular_pulse_positions(jittered, form, monkeypatch):
+    """The quadratic carries only R_q(x) - R_q(x0), which moves by |x - x0|/R
+    (<6e-4 here) of a pulse-position error: ~1e-6 m for this jitter, more
+    than the fit's 1e-7 m end checks allow, so the waves keep the FP32 chain
+    (the two forms give the same result); either way within the bounds."""
+    if form == "chain":
+        monkeypatch.setenv("TDBP_MIXED_CHAIN", "1")
+    d, ref = jittered
+    d.fit()
+    m = ref["azoff"] > 0
+    for dtype in (T.CF32, T.CF16):
+        got = d.run(T.MIXED, dtype)
+        ph, amp = errors(got, ref["out"], m)
+        print(f"jittered {form} {T.DTYPE_NAMES[dtype]}: phase {ph*1e3:.4f} mrad, amp {amp:.3e}")
+        pb, ab = BOUNDS[dtype]
+        assert ph <= pb and amp <= ab
+        assert np.all(got[~m] == 0)
+
+
+@pytest.mark.parametrize("scene", ["c1", "fbm_small"])
+def test_quadratic_form_matches_chain(scene, request, monkeypatch):
+    """On cf32 samples the quadratic form and the FP32 chain agree to 2e-5
+    rad and 2e-5 in amplitude: both evaluate the same range to ~1e-7 m."""
+    d, ref = request.getfixturevalue(scene)
+    d.fit()
+    m = ref["azoff"] > 0
+    quad = d.run(T.MIXED, T.CF32)
+    monkeypatch.setenv("TDBP_MIXED_CHAIN", "1")
+    chain = d.run(T.MIXED, T.CF32)
+    ph, amp = errors(quad, chain, m)
+    print(f"{scene} quadratic vs chain: phase {ph:.3e} rad, amp {amp:.3e}")
+    assert ph <= 2e-5 and amp <= 2e-5
 
 
 MFMA_VARIANT = 8   # the MFMA kernel: range index and phase from v_mfma_f32_16x16x32_f16
