@@ -518,38 +518,75 @@ __device__ __forceinline__ void stage_pulses(const MixedPixel& m, const PolyPixe
   };
   for (int qq = qa; qq < ea; ++qq) checked(qq);
   if (stage_safe) {
-    // No tests.  Two pairs in flight: each sample load is issued a whole pair
-    // ahead of its use (fa and fb keep their own registers; the scheduling
-    // barrier after each front keeps the load there).  An odd count starts
-    // with one plain pair; an empty core (cb <= ca) does nothing.  Measured
-    // with the FP32 chain: one pair in flight 1240, two 1442-1546, three 1%
-    // below two Gpixel-pulse/s on the 27-burst IW scene.
+    // No tests.  Each sample load is issued ahead of its use (the fronts keep
+    // their own registers; the scheduling barrier after each front keeps the
+    // load there).  An empty core (cb <= ca) does nothing.
     int qq = ca;
-    if (qq < cb && ((cb - qq) & 1)) {
-      unchecked(qq);
-      ++qq;
-    }
-    if (qq < cb) {
-      grow_t rp = base + (size_t)qq * rowbytes;
-      unsigned so = (unsigned)((long)qq * (long)rowbytes + shift + kSbufBias);
-      const unsigned rb32 = (unsigned)rowbytes;
-      const PulseRec* rr = rec + (qq - p0);
-      PairFront<DT> fa = mixed_front<DT, POLY>(m, pp, rr[0], rp, rsrc, so, Kf, invd);
-      PairFront<DT> fb;
-      for (; qq + 2 < cb; qq += 2) {
-        fb = mixed_front<DT, POLY>(m, pp, rr[1], rp + rowbytes, rsrc, so + rb32, Kf, invd);
-        __builtin_amdgcn_sched_barrier(0);
+    if constexpr (POLY) {
+      // Three pairs in flight: the load of pulse q + 2 is issued before the
+      // back of pulse q.  (cb - ca) mod 3 plain pairs first; the rest is a
+      // multiple of 3, so a loop pass (taken when >= 6 remain) reads records
+      // q..q+4 inside the core.  Measured: 2.6 % above two in flight on the
+      // 27-burst IW scene (the quadratic's shorter body leaves less work to
+      // cover the gather with).
+      const int rem = (cb - qq) % 3;
+      for (int r = 0; r < rem; ++r) unchecked(qq++);
+      if (qq < cb) {
+        grow_t rp = base + (size_t)qq * rowbytes;
+        unsigned so = (unsigned)((long)qq * (long)rowbytes + shift + kSbufBias);
+        const unsigned rb32 = (unsigned)rowbytes;
+        const PulseRec* rr = rec + (qq - p0);
+        PairFront<DT> fa = mixed_front<DT, POLY>(m, pp, rr[0], rp, rsrc, so, Kf, invd);
+        PairFront<DT> fb = mixed_front<DT, POLY>(m, pp, rr[1], rp, rsrc, so + rb32, Kf, invd);
+        PairFront<DT> fc;
+        for (; qq + 3 < cb; qq += 3) {
+          fc = mixed_front<DT, POLY>(m, pp, rr[2], rp, rsrc, so + 2 * rb32, Kf, invd);
+          __builtin_amdgcn_sched_barrier(0);
+          mixed_back<DT>(fa, sr, si);
+          fa = mixed_front<DT, POLY>(m, pp, rr[3], rp, rsrc, so + 3 * rb32, Kf, invd);
+          __builtin_amdgcn_sched_barrier(0);
+          mixed_back<DT>(fb, sr, si);
+          fb = mixed_front<DT, POLY>(m, pp, rr[4], rp, rsrc, so + 4 * rb32, Kf, invd);
+          __builtin_amdgcn_sched_barrier(0);
+          mixed_back<DT>(fc, sr, si);
+          so += 3 * rb32;
+          rr += 3;
+        }
+        fc = mixed_front<DT, POLY>(m, pp, rr[2], rp, rsrc, so + 2 * rb32, Kf, invd);
         mixed_back<DT>(fa, sr, si);
-        rp += 2 * rowbytes;
-        so += 2 * rb32;
-        rr += 2;
-        fa = mixed_front<DT, POLY>(m, pp, rr[0], rp, rsrc, so, Kf, invd);
-        __builtin_amdgcn_sched_barrier(0);
+        mixed_back<DT>(fb, sr, si);
+        mixed_back<DT>(fc, sr, si);
+      }
+    } else {
+      // FP32 chain: two pairs in flight; an odd count starts with one plain
+      // pair.  Measured: one pair in flight 1240, two 1442-1546, three 1% below
+      // two Gpixel-pulse/s on the 27-burst IW scene.
+      if (qq < cb && ((cb - qq) & 1)) {
+        unchecked(qq);
+        ++qq;
+      }
+      if (qq < cb) {
+        grow_t rp = base + (size_t)qq * rowbytes;
+        unsigned so = (unsigned)((long)qq * (long)rowbytes + shift + kSbufBias);
+        const unsigned rb32 = (unsigned)rowbytes;
+        const PulseRec* rr = rec + (qq - p0);
+        PairFront<DT> fa = mixed_front<DT, POLY>(m, pp, rr[0], rp, rsrc, so, Kf, invd);
+        PairFront<DT> fb;
+        for (; qq + 2 < cb; qq += 2) {
+          fb = mixed_front<DT, POLY>(m, pp, rr[1], rp + rowbytes, rsrc, so + rb32, Kf, invd);
+          __builtin_amdgcn_sched_barrier(0);
+          mixed_back<DT>(fa, sr, si);
+          rp += 2 * rowbytes;
+          so += 2 * rb32;
+          rr += 2;
+          fa = mixed_front<DT, POLY>(m, pp, rr[0], rp, rsrc, so, Kf, invd);
+          __builtin_amdgcn_sched_barrier(0);
+          mixed_back<DT>(fb, sr, si);
+        }
+        fb = mixed_front<DT, POLY>(m, pp, rr[1], rp + rowbytes, rsrc, so + rb32, Kf, invd);
+        mixed_back<DT>(fa, sr, si);
         mixed_back<DT>(fb, sr, si);
       }
-      fb = mixed_front<DT, POLY>(m, pp, rr[1], rp + rowbytes, rsrc, so + rb32, Kf, invd);
-      mixed_back<DT>(fa, sr, si);
-      mixed_back<DT>(fb, sr, si);
     }
   } else {
     for (int qq = ca; qq < cb; ++qq) {
@@ -1298,10 +1335,7 @@ __global__ __launch_bounds__(kBlock) void k_plain(BurstDev b) {
 // Occupancy target of the mixed kernels: 8 waves per SIMD, the hardware
 // maximum (<= 64 VGPRs); the pulse loop is a dependent gather, so resident
 // waves hide its latency (measured: 7 -> 8 waves is +9 % on the 27-burst IW scene).
-#ifndef TDBP_MIXED_WAVES
-#define TDBP_MIXED_WAVES 8
-#endif
-#define TDBP_MIXED_OCC __attribute__((amdgpu_waves_per_eu(TDBP_MIXED_WAVES)))
+#define TDBP_MIXED_OCC __attribute__((amdgpu_waves_per_eu(8)))
 
 template <int TR, bool STACK, int DT>
 __global__ __launch_bounds__(kBlock) TDBP_MIXED_OCC void k_mixed(BurstDev b) {
