@@ -703,17 +703,12 @@ def main():
         batch.close()
         return
 
-    # Autotune the mixed kernel's tile shape on the largest burst (or take
-    # the variant given).
+    # The mixed kernel's tile shape: the variant given, or the batch
+    # autotuner's pick (every VALU variant timed on this batch of bursts; it
+    # also becomes the default of the other-storage batches below).
     if args.variant >= 0:
         T.lib().tdbp_set_variant(args.variant)
         log(f"variant {args.variant} {T.variant_info(args.variant)} (given)")
-    else:
-        big = max(dev, key=lambda d: d["sc"].geom.nlines * d["sc"].geom.demwidth)
-        best, tune_ms = T.autotune(big["sc"].geom, smp(big, stor), big["sat"], big["dem"],
-                                   big["out"], T.MIXED, reps=2)
-        log(f"autotune: variant {best} {T.variant_info(best)}; "
-            f"ms per variant {[round(x, 2) for x in tune_ms]}")
 
     ev0, ev1 = T.Event(), T.Event()
     stream = T.Stream()
@@ -744,6 +739,10 @@ def main():
         return b
 
     batch = make_batch(stor)
+    if args.variant < 0:
+        best, tune_ms = batch.autotune(reps=2)
+        log(f"batch autotune: variant {best} {T.variant_info(best)}; "
+            f"ms per variant {[round(x, 2) for x in tune_ms]}")
     batch.run(stream)
     stream.sync()
     pairs = batch.pairs

@@ -304,6 +304,14 @@ int tdbp_batch_set_merge(tdbp_batch *b, int burst, void *d_grid, int grid_lines,
 int tdbp_batch_run(tdbp_batch *b, void *stream);
 /* Variant the batch runs (resolved at the first run). */
 int tdbp_batch_variant(const tdbp_batch *b);
+/* Time every VALU variant (0-7) on this batch's own bursts -- reps graph
+ * launches each after a warm-up -- and keep the fastest for the batch and as
+ * the default of later TDBP_MIXED launches (tdbp_set_variant).  ms (optional,
+ * [tdbp_variant_count()]) gets the mean milliseconds per launch of each
+ * variant (0 for the MFMA variant 8).  Overwrites the outputs and re-applies
+ * the merges (an overwrite, so the grid ends as after one run).  No
+ * counterpart in the reference: the batched path is new. */
+int tdbp_batch_autotune(tdbp_batch *b, int reps, float *ms, int *best);
 /* Pairs visited by one run (valid after the first run). */
 int64_t tdbp_batch_pairs(const tdbp_batch *b);
 /* Device memory held by the batch itself (queue, descriptors, coef). */

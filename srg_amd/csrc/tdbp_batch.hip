@@ -31,14 +31,15 @@ struct tdbp_batch {
   int dtype = -1;
   std::vector<BurstDev> bursts;
   std::vector<MergeTarget> merges;  // per burst, applied in burst order
-  std::vector<void*> owned;  // device allocations (orbits, coefs, tables)
+  std::vector<void*> owned;  // device allocations of the bursts (orbits, coefs)
+  std::vector<void*> tables; // device allocations of build() (burst table, queues)
   BurstDev* d_bursts = nullptr;
   int2* d_items = nullptr;
   int* d_qbase = nullptr;
   int* d_counters = nullptr;
   int nitems = 0, grid = 0, max_lines = 0;
   int64_t pairs = 0;
-  size_t dev_bytes = 0;
+  size_t dev_bytes = 0, table_bytes = 0;
   hipStream_t cap = nullptr;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
@@ -53,6 +54,37 @@ int balloc(tdbp_batch* b, void** p, size_t bytes) {
   b->owned.push_back(*p);
   b->dev_bytes += bytes;
   return TDBP_OK;
+}
+
+int talloc(tdbp_batch* b, void** p, size_t bytes) {
+  if (hipMalloc(p, bytes ? bytes : 1) != hipSuccess)
+    return set_error(TDBP_ERR_NOMEM, "batch hipMalloc(%zu) failed", bytes);
+  b->tables.push_back(*p);
+  b->dev_bytes += bytes;
+  b->table_bytes += bytes;
+  return TDBP_OK;
+}
+
+// Undo build(): the graph, the stream and the per-variant tables.
+void unbuild(tdbp_batch* b) {
+  if (b->cap) (void)hipStreamSynchronize(b->cap);
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  if (b->cap) (void)hipStreamDestroy(b->cap);
+  b->exec = nullptr;
+  b->graph = nullptr;
+  b->cap = nullptr;
+  for (void* p : b->tables) (void)hipFree(p);
+  b->tables.clear();
+  b->d_bursts = nullptr;
+  b->d_items = nullptr;
+  b->d_qbase = nullptr;
+  b->d_counters = nullptr;
+  b->nitems = 0;
+  b->grid = 0;
+  b->dev_bytes -= b->table_bytes;
+  b->table_bytes = 0;
+  b->built = false;
 }
 
 int enqueue(tdbp_batch* b, hipStream_t s) {
@@ -97,7 +129,7 @@ int build(tdbp_batch* b) {
     b->max_lines = std::max(b->max_lines, bd.nlines);
   }
   const size_t nb = b->bursts.size();
-  if ((st = balloc(b, (void**)&b->d_bursts, nb * sizeof(BurstDev)))) return st;
+  if ((st = talloc(b, (void**)&b->d_bursts, nb * sizeof(BurstDev)))) return st;
   if ((st = hip_check(hipMemcpy(b->d_bursts, b->bursts.data(),
                                 nb * sizeof(BurstDev), hipMemcpyHostToDevice),
                       "burst table upload")))
@@ -110,7 +142,7 @@ int build(tdbp_batch* b) {
   size_t tot_lines = 0;
   for (auto& bd : b->bursts) tot_lines += bd.nlines;
   int64_t* d_pairs = nullptr;
-  if ((st = balloc(b, (void**)&d_pairs, tot_lines * sizeof(int64_t)))) return st;
+  if ((st = talloc(b, (void**)&d_pairs, tot_lines * sizeof(int64_t)))) return st;
   size_t off = 0;
   for (auto& bd : b->bursts) {
     if ((st = hip_check(launch_pair_count(bd, d_pairs + off, b->cap),
@@ -146,9 +178,9 @@ int build(tdbp_batch* b) {
   }
   qbase[8] = (int)items.size();
   b->nitems = (int)items.size();
-  if ((st = balloc(b, (void**)&b->d_items, items.size() * sizeof(int2))) ||
-      (st = balloc(b, (void**)&b->d_qbase, 9 * sizeof(int))) ||
-      (st = balloc(b, (void**)&b->d_counters, 8 * sizeof(int))))
+  if ((st = talloc(b, (void**)&b->d_items, items.size() * sizeof(int2))) ||
+      (st = talloc(b, (void**)&b->d_qbase, 9 * sizeof(int))) ||
+      (st = talloc(b, (void**)&b->d_counters, 8 * sizeof(int))))
     return st;
   if ((st = hip_check(hipMemcpy(b->d_items, items.data(),
                                 items.size() * sizeof(int2), hipMemcpyHostToDevice),
@@ -275,15 +307,59 @@ int tdbp_batch_variant(const tdbp_batch* b) { return b ? b->variant : -1; }
 
 void tdbp_batch_destroy(tdbp_batch* b) {
   if (!b) return;
-  if (b->exec) (void)hipGraphExecDestroy(b->exec);
-  if (b->graph) (void)hipGraphDestroy(b->graph);
-  if (b->cap) {
-    (void)hipStreamSynchronize(b->cap);
-    (void)hipStreamDestroy(b->cap);
-  }
+  unbuild(b);
   (void)hipDeviceSynchronize();
   for (void* p : b->owned) (void)hipFree(p);
   delete b;
+}
+
+// Time every VALU variant on the batch's own bursts and keep the fastest.
+int tdbp_batch_autotune(tdbp_batch* b, int reps, float* ms, int* best) {
+  if (!b) return set_error(TDBP_ERR_ARG, "null batch");
+  if (b->bursts.empty()) return set_error(TDBP_ERR_ARG, "empty batch");
+  int st = current_device_ready();
+  if (st) return st;
+  reps = reps < 1 ? 1 : reps;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if ((st = hip_check(hipEventCreate(&e0), "event")) ||
+      (st = hip_check(hipEventCreate(&e1), "event"))) {
+    if (e0) (void)hipEventDestroy(e0);
+    return st;
+  }
+  int bestv = -1;
+  float bestms = 3.4e38f;
+  for (int v = 0; v < kNumVariants && !st; ++v) {
+    // The MFMA variant (8) runs only when asked for by index.
+    if (kVariants[v].mfma) {
+      if (ms) ms[v] = 0.f;
+      continue;
+    }
+    unbuild(b);
+    b->variant = v;
+    if ((st = build(b))) break;
+    if ((st = tdbp_batch_run(b, b->cap))) break;   // warm-up
+    (void)hipEventRecord(e0, b->cap);
+    for (int r = 0; r < reps && !st; ++r) st = tdbp_batch_run(b, b->cap);
+    (void)hipEventRecord(e1, b->cap);
+    if (!st) st = hip_check(hipEventSynchronize(e1), "tune sync");
+    float t = 0.f;
+    if (!st) (void)hipEventElapsedTime(&t, e0, e1);
+    t /= reps;
+    if (ms) ms[v] = t;
+    if (!st && t < bestms) {
+      bestms = t;
+      bestv = v;
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  unbuild(b);
+  if (st) return st;
+  b->variant = bestv;
+  if ((st = build(b))) return st;
+  if (best) *best = bestv;
+  if (b->precision == TDBP_MIXED) tdbp_set_variant(bestv);
+  return TDBP_OK;
 }
 
 // Time every variant on one burst and keep the fastest for later TDBP_MIXED

@@ -183,6 +183,8 @@ def _sig(lib):
                                      _VP]),
         "tdbp_batch_pairs": (C.c_int64, [_VP]),
         "tdbp_batch_variant": (C.c_int, [_VP]),
+        "tdbp_batch_autotune": (C.c_int, [_VP, C.c_int, C.POINTER(C.c_float),
+                                          C.POINTER(C.c_int)]),
         "tdbp_batch_device_bytes": (C.c_size_t, [_VP]),
         "tdbp_batch_destroy": (None, [_VP]),
         "tdbp_synth_clutter": (C.c_int, [_VP, C.c_int, C.c_int, C.c_uint64, C.c_float,
@@ -214,7 +216,7 @@ def exported_symbols():
         "tdbp_backproject_variant", "tdbp_autotune", "tdbp_set_variant",
         "tdbp_variant_count", "tdbp_variant_info", "tdbp_batch_create", "tdbp_batch_add",
         "tdbp_batch_run", "tdbp_batch_set_merge", "tdbp_merge_geo", "tdbp_compare", "tdbp_dropin_timing", "tdbp_batch_pairs",
-        "tdbp_batch_variant",
+        "tdbp_batch_variant", "tdbp_batch_autotune",
         "tdbp_batch_device_bytes", "tdbp_batch_destroy", "tdbp_synth_clutter",
         "tdbp_synth_targets", "azimuth_compress_", "azimuth_compress_cpu_",
     ]
@@ -708,6 +710,15 @@ class Batch:
 
     def run(self, stream=None):
         check(lib().tdbp_batch_run(self.ptr, _s(stream)))
+
+    def autotune(self, reps: int = 2):
+        """Time every VALU variant on this batch (tdbp_batch_autotune) and keep
+        the fastest; returns (best, ms per variant)."""
+        n = lib().tdbp_variant_count()
+        ms = (C.c_float * n)()
+        best = C.c_int(-1)
+        check(lib().tdbp_batch_autotune(self.ptr, reps, ms, C.byref(best)))
+        return best.value, list(ms)
 
     @property
     def pairs(self) -> int:

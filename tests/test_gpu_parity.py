@@ -250,6 +250,47 @@ def test_all_mixed_variants_agree(c1):
         assert ph <= 1e-6 and amp <= 1e-6, (v, ph, amp)
 
 
+def test_batch_autotune(gpu):
+    """tdbp_batch_autotune times variants 0-7 on the batch, keeps the fastest
+    (and makes it the default), and the batch then gives the outputs of a
+    fresh batch built with that variant, bit for bit."""
+    src = synth.fbm_surface((256, 256), seed=6)
+    scenes = [synth.make_burst(L, S, W, nl, dem="fbm", dem_source=src, seed=60 + i,
+                               ntargets=2, centre_range=840e3 + 25e3 * i, aperture=180)
+              for i, (L, S, W, nl) in enumerate([(420, 3200, 240, 110), (500, 3900, 180, 95)])]
+    devs = [Dev(sc) for sc in scenes]
+    packed = [T.pack(d.d_data, d.sc.L, d.sc.S, T.CF16) for d in devs]
+
+    def batch(v):
+        b = T.Batch(T.MIXED, v)
+        outs = []
+        for d, (pk, sc) in zip(devs, packed):
+            o = T.DeviceArray((d.sc.geom.nlines, d.sc.geom.demwidth), np.complex64)
+            o.zero()
+            outs.append(o)
+            b.add(d.sc.geom, T.samples(pk, T.CF16, sc), d.d_sat, d.d_dem, o)
+        return b, outs
+
+    b, outs = batch(-1)
+    best, ms = b.autotune(reps=1)
+    assert 0 <= best < 8 and b.variant == best
+    assert ms[8] == 0.0 and all(t > 0 for t in ms[:8])
+    assert ms[best] == min(ms[:8])
+    for o in outs:
+        o.zero()
+    b.run()
+    T.sync()
+    got = [o.download() for o in outs]
+    b2, outs2 = batch(best)
+    b2.run()
+    T.sync()
+    for g, o in zip(got, outs2):
+        assert np.array_equal(g, o.download())
+    b.close()
+    b2.close()
+    T.lib().tdbp_set_variant(-1)
+
+
 def test_plain_is_deterministic(fbm_small):
     d, _ = fbm_small
     d.fit()
