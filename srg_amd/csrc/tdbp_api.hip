@@ -208,7 +208,7 @@ int default_variant(int precision) {
   }
   if (g_variant >= 0) return g_variant;
   const char* e = std::getenv("TDBP_VARIANT");
-  const int v = e ? std::atoi(e) : 4;
+  const int v = e ? std::atoi(e) : 5;
   return (v >= 0 && v < kNumVariants) ? v : 4;
 }
 
