@@ -26,6 +26,7 @@
 #include <hip/hip_runtime.h>
 #include <unistd.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -51,7 +52,7 @@ namespace {
 // does that burst's .geo read-merge-write, in call order, while the next call
 // uploads into the other slot.
 enum { kBurst, kPacked, kScale, kSat, kDem, kCoef, kOut, kGeo, kNumDev };
-enum { kHostDem, kHostGeo, kNumHost };
+enum { kHostDem, kHostGeo, kHostStage0, kHostStage1, kNumHost };
 struct Slot {
   void* p[kNumDev] = {};
   size_t cap[kNumDev] = {};
@@ -59,7 +60,72 @@ struct Slot {
   size_t hcap[kNumHost] = {};
   hipStream_t s = nullptr;
   hipEvent_t ev[6] = {};
+  hipEvent_t sev[2] = {};   // the last DMA out of each pinned staging chunk
   bool busy = false;   // an asynchronous .geo job still uses this slot
+};
+
+// Host copies split over the calling thread and helper threads (one part
+// each), for filling the pinned staging chunks of the burst upload.  The
+// helpers are detached and wait on a condition variable between copies.
+class CopyPool {
+ public:
+  static CopyPool& get() {
+    static CopyPool* p = new CopyPool();   // never destroyed: helpers may outlive main
+    return *p;
+  }
+  void copy(void* dst, const void* src, size_t n) {
+    const int parts = nhelp_ + 1;
+    if (nhelp_ == 0 || n < ((size_t)1 << 20)) {
+      std::memcpy(dst, src, n);
+      return;
+    }
+    {
+      std::lock_guard<std::mutex> lk(mu_);
+      dst_ = (char*)dst;
+      src_ = (const char*)src;
+      n_ = n;
+      parts_ = parts;
+      left_ = nhelp_;
+      ++gen_;
+    }
+    cv_.notify_all();
+    part(0);
+    std::unique_lock<std::mutex> lk(mu_);
+    done_.wait(lk, [&] { return left_ == 0; });
+  }
+
+ private:
+  CopyPool() {
+    const unsigned hc = std::thread::hardware_concurrency();
+    nhelp_ = hc >= 4 ? (int)std::min(hc / 2, 8u) - 1 : 0;
+    for (int i = 1; i <= nhelp_; ++i) std::thread([this, i] { loop(i); }).detach();
+  }
+  void part(int i) {
+    // 64-byte aligned split points so helpers do not share cache lines.
+    const size_t per = ((n_ / parts_) + 63) & ~(size_t)63;
+    const size_t a = std::min(n_, per * i), b = std::min(n_, per * (i + 1));
+    if (b > a) std::memcpy(dst_ + a, src_ + a, b - a);
+  }
+  void loop(int i) {
+    unsigned long long seen = 0;
+    for (;;) {
+      {
+        std::unique_lock<std::mutex> lk(mu_);
+        cv_.wait(lk, [&] { return gen_ != seen; });
+        seen = gen_;
+      }
+      part(i);
+      std::lock_guard<std::mutex> lk(mu_);
+      if (--left_ == 0) done_.notify_one();
+    }
+  }
+  std::mutex mu_;
+  std::condition_variable cv_, done_;
+  unsigned long long gen_ = 0;
+  char* dst_ = nullptr;
+  const char* src_ = nullptr;
+  size_t n_ = 0;
+  int parts_ = 1, left_ = 0, nhelp_ = 0;
 };
 
 // One pending .geo read-merge-write.
@@ -119,6 +185,31 @@ int need_host(Slot& sl, int k, size_t bytes) {
     return set_error(TDBP_ERR_NOMEM, "drop-in hipHostMalloc(%zu) failed", bytes);
   }
   sl.hcap[k] = bytes;
+  return TDBP_OK;
+}
+
+// The burst (pageable caller memory) to the device through two pinned
+// chunks: the copy threads fill one chunk while the DMA engine drains the
+// other.  Returns once every byte has left the caller's buffer.
+constexpr size_t kStageChunk = (size_t)32 << 20;
+
+int upload_staged(Slot& sl, void* d_dst, const void* h_src, size_t bytes, hipStream_t s) {
+  int st;
+  if ((st = need_host(sl, kHostStage0, kStageChunk)) ||
+      (st = need_host(sl, kHostStage1, kStageChunk)))
+    return st;
+  for (size_t off = 0, c = 0; off < bytes; off += kStageChunk, ++c) {
+    const int k = (int)(c & 1);
+    const size_t n = std::min(kStageChunk, bytes - off);
+    // The chunk's previous DMA (this call's chunk c-2, or an earlier call's).
+    if ((st = hip_check(hipEventSynchronize(sl.sev[k]), "staging wait"))) return st;
+    CopyPool::get().copy(sl.h[kHostStage0 + k], (const char*)h_src + off, n);
+    if ((st = hip_check(hipMemcpyAsync((char*)d_dst + off, sl.h[kHostStage0 + k], n,
+                                       hipMemcpyHostToDevice, s),
+                        "burst upload")) ||
+        (st = hip_check(hipEventRecord(sl.sev[k], s), "staging event")))
+      return st;
+  }
   return TDBP_OK;
 }
 
@@ -354,6 +445,9 @@ int run(const float* burst, const double* satloc, int L, int S, int W,
       return st;
     for (auto& e : sl.ev)
       if ((st = hip_check(hipEventCreate(&e), "drop-in event"))) return st;
+    for (auto& e : sl.sev)
+      if ((st = hip_check(hipEventCreateWithFlags(&e, hipEventDisableTiming), "drop-in event")))
+        return st;
   }
   hipStream_t s = sl.s;
   hipEvent_t* ev = sl.ev;
@@ -381,9 +475,7 @@ int run(const float* burst, const double* satloc, int L, int S, int W,
       (st = hip_check(hipMemcpyAsync(sl.p[kSat], satloc, (size_t)L * 24,
                                      hipMemcpyHostToDevice, s),
                       "satloc upload")) ||
-      (st = hip_check(hipMemcpyAsync(sl.p[kBurst], burst, nsamp * 8, hipMemcpyHostToDevice,
-                                     s),
-                      "burst upload")))
+      (st = upload_staged(sl, sl.p[kBurst], burst, nsamp * 8, s)))
     return st;
   (void)hipEventRecord(ev[1], s);
   tdbp_samples smp;
