@@ -112,6 +112,48 @@ TDBP_HD double zero_doppler_range(const double* xyz, const double* t,
 
 // WGS-84 geodetic -> ECEF of DEM sample (window line `line`, column `pixel`)
 // as in sentinel/azimuth_compress.cu:117-131.
+// sqrt x from an estimate y0 of 1/sqrt x (relative error up to ~1e-7) for
+// normal x > 0: Goldschmidt's iteration (g -> sqrt x, h -> 1/(2 sqrt x))
+// and two residual corrections; correctly rounded on the host checks.
+TDBP_HD double goldschmidt_sqrt(double x, double y0) {
+  double g = x * y0, h = 0.5 * y0;
+  const double r = fma(-h, g, 0.5);
+  g = fma(g, r, g);
+  h = fma(h, r, h);
+  double d = fma(-g, g, x);
+  g = fma(d, h, g);
+  d = fma(-g, g, x);
+  return fma(d, h, g);
+}
+
+// sin and cos of 2 pi f for |f| <= 1/2 in FP64: t = 4f = n + d exactly
+// (n = rint t), th = d pi/2 in [-pi/4, pi/4], Taylor series to th^13 (sin)
+// and th^12 (cos), truncation below 1e-11; the quadrant n mod 4 swaps and
+// negates.
+TDBP_HD void sincos_2pi(double f, double* sn, double* cs) {
+  const double t = 4.0 * f;
+  const double n = rint(t);
+  const double th = (t - n) * 1.5707963267948966;
+  const double z = th * th;
+  double ps = -1.0 / 39916800.0;
+  ps = fma(z, ps, 1.0 / 362880.0);
+  ps = fma(z, ps, -1.0 / 5040.0);
+  ps = fma(z, ps, 1.0 / 120.0);
+  ps = fma(z, ps, -1.0 / 6.0);
+  const double s0 = fma(th * z, ps, th);
+  double pc = 1.0 / 479001600.0;
+  pc = fma(z, pc, -1.0 / 3628800.0);
+  pc = fma(z, pc, 1.0 / 40320.0);
+  pc = fma(z, pc, -1.0 / 720.0);
+  pc = fma(z, pc, 1.0 / 24.0);
+  pc = fma(z, pc, -0.5);
+  const double c0 = fma(z, pc, 1.0);
+  const int q = (int)n & 3;   // angle = q pi/2 + th
+  const double sa = (q & 1) ? c0 : s0, ca = (q & 1) ? s0 : c0;
+  *sn = (q & 2) ? -sa : sa;
+  *cs = ((q + 1) & 2) ? -ca : ca;
+}
+
 TDBP_HD void dem_to_ecef(int line, int firstline, int pixel, double firstlat,
                          double deltalat, double firstlon, double deltalon,
                          double hgt, double* o) {

@@ -129,46 +129,9 @@ __device__ __forceinline__ void store_out(const BurstDev& b, const PixelCtx& p,
 // One pixel-pulse pair in FP64: range, gate, FP32 lerp of the two samples,
 // phase exp(+j 4 pi R / lambda) reduced in cycles before sincospi.
 // sqrt of a range squared (normal, ~1e12 m^2): the hardware reciprocal-root
-// estimate refined by Goldschmidt's iteration (g -> sqrt x, h -> 1/(2 sqrt x)),
-// two residual corrections; within an ulp of the correctly rounded root.
+// estimate refined by goldschmidt_sqrt.
 __device__ __forceinline__ double range_sqrt(double x) {
-  const double y = __builtin_amdgcn_rsq(x);
-  double g = x * y, h = 0.5 * y;
-  const double r = fma(-h, g, 0.5);
-  g = fma(g, r, g);
-  h = fma(h, r, h);
-  double d = fma(-g, g, x);
-  g = fma(d, h, g);
-  d = fma(-g, g, x);
-  return fma(d, h, g);
-}
-
-// sin and cos of 2 pi f for |f| <= 1/2 in FP64: t = 4f = n + d exactly
-// (n = rint t), th = d pi/2 in [-pi/4, pi/4], Taylor series to th^13 (sin)
-// and th^12 (cos), truncation below 1e-11; the quadrant n mod 4 swaps and
-// negates.
-__device__ __forceinline__ void sincos_2pi(double f, double* sn, double* cs) {
-  const double t = 4.0 * f;
-  const double n = rint(t);
-  const double th = (t - n) * 1.5707963267948966;
-  const double z = th * th;
-  double ps = -1.0 / 39916800.0;
-  ps = fma(z, ps, 1.0 / 362880.0);
-  ps = fma(z, ps, -1.0 / 5040.0);
-  ps = fma(z, ps, 1.0 / 120.0);
-  ps = fma(z, ps, -1.0 / 6.0);
-  const double s0 = fma(th * z, ps, th);
-  double pc = 1.0 / 479001600.0;
-  pc = fma(z, pc, -1.0 / 3628800.0);
-  pc = fma(z, pc, 1.0 / 40320.0);
-  pc = fma(z, pc, -1.0 / 720.0);
-  pc = fma(z, pc, 1.0 / 24.0);
-  pc = fma(z, pc, -0.5);
-  const double c0 = fma(z, pc, 1.0);
-  const int q = (int)n & 3;   // angle = q pi/2 + th
-  const double sa = (q & 1) ? c0 : s0, ca = (q & 1) ? s0 : c0;
-  *sn = (q & 2) ? -sa : sa;
-  *cs = ((q + 1) & 2) ? -ca : ca;
+  return goldschmidt_sqrt(x, __builtin_amdgcn_rsq(x));
 }
 
 __device__ __forceinline__ void plain_pair(const float2* __restrict__ row,
