@@ -517,21 +517,13 @@ def stack_bench(args, synth, T):
         stream.sync()
         plain["ms"] += e0.elapsed_ms(e1)
 
-    tuned = {}
-
-    def tune(idx, res):
-        if tuned:
-            return
-        k = max(range(len(idx)), key=lambda j: bursts[idx[j]].geom.nlines)
-        g = bursts[idx[k]].geom
-        r = res[k]
-        best, ms = T.autotune(g, r.samples, r.sat, r.dem, r.out, T.MIXED, reps=2)
-        tuned["variant"] = best
-        log(f"autotune: variant {best} {T.variant_info(best)}")
-
     budget = int(args.hbm_budget * 2**30) if args.hbm_budget else None
     run = ST.run_stack(bursts, g_mixed, loader, precision=T.MIXED, dtype=T.CF16,
-                       budget=budget, repeats=args.warmup + args.steps, on_resident=tune)
+                       budget=budget, repeats=args.warmup + args.steps, autotune_reps=2)
+    tuned = {"variant": run.tuned[0]} if run.tuned else {}
+    if run.tuned:
+        log(f"batch autotune: variant {run.tuned[0]} {T.variant_info(run.tuned[0])}; "
+            f"ms per variant {[round(x, 1) for x in run.tuned[1]]}")
     pairs = sum(run.pairs)
     per_batch = [float(np.mean(ms[args.warmup:])) for ms in run.run_ms]
     step_ms = sum(per_batch)

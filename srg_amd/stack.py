@@ -119,6 +119,7 @@ class StackRun:
     batches: List[List[int]]
     run_ms: List[List[float]] = field(default_factory=list)
     pairs: List[int] = field(default_factory=list)
+    tuned: Optional[tuple] = None   # (variant, ms per variant) of the batch autotuner
 
 
 def allocate(burst: StackBurst, dtype: int) -> Resident:
@@ -153,7 +154,7 @@ def run_stack(bursts: Sequence[StackBurst], grids: Sequence[tdbp.DeviceArray],
               loader: Loader, *, precision: int = tdbp.MIXED, dtype: int = tdbp.CF16,
               variant: int = -1, budget: Optional[int] = None, repeats: int = 1,
               on_resident: Optional[Callable[[List[int], List[Resident]], None]] = None,
-              ) -> StackRun:
+              autotune_reps: int = 0) -> StackRun:
     """Backproject every burst into its acquisition grid.
 
     Bursts are admitted in order into batches that fit `budget` device bytes
@@ -162,7 +163,11 @@ def run_stack(bursts: Sequence[StackBurst], grids: Sequence[tdbp.DeviceArray],
     merge of each burst into grids[burst.acq] at its first DEM line, run it
     `repeats` times (device time of each run recorded), free the batch.
     `on_resident(indices, residents)` is called after the loads, before the
-    batch runs.  The grids must be zeroed by the caller once before the stack.
+    batch runs.  With autotune_reps > 0 (and variant -1) the first batch times
+    every VALU variant on its own bursts (tdbp_batch_autotune) and the pick
+    serves every batch; the merges it re-applies are overwrites of the same
+    rows, so the grids end as after the timed runs alone.  The grids must be
+    zeroed by the caller once before the stack.
     """
     if precision == tdbp.PLAIN_FP64 and dtype != tdbp.CF32:
         raise ValueError("the FP64 kernel reads cf32 samples")
@@ -192,6 +197,8 @@ def run_stack(bursts: Sequence[StackBurst], grids: Sequence[tdbp.DeviceArray],
                     g = bursts[i].geom
                     batch.add(g, r.samples, r.sat, r.dem, r.out)
                     batch.set_merge(k, grids[bursts[i].acq], g.firstline)
+                if autotune_reps > 0 and variant < 0 and run.tuned is None:
+                    run.tuned = batch.autotune(autotune_reps)
                 times = []
                 for _ in range(repeats):
                     ev0.record(stream)
