@@ -8,7 +8,11 @@
 //   * each row times `scale`, zero-padded to nnn = the next power of two >= na,
 //     forward FFT; the spectrum halves placed at both ends of a 2 nnn array;
 //     inverse FFT; divided by nnn, then by 2: the row upsampled by two
-//     (:127-148);
+//     (:127-148).  Computed here without the 2 nnn transform: with X the
+//     spectrum of the scaled row x, the even samples of that result are x/2
+//     exactly, and the odd ones the length-nnn inverse transform of X times
+//     exp(j pi k/nnn) (k signed, the upper half of X at k - nnn, as the
+//     placement puts it), divided by nnn and by 2;
 //   * igram = up1 conj(up2), amp = (|up1|^2, |up2|^2) over the first 2 na
 //     samples; adjacent pairs summed back to na columns; summed over the
 //     looksdn rows, then over looksac columns; amp square-rooted per
@@ -48,33 +52,35 @@ __global__ void k_up_load(const float2* __restrict__ img, int na, int nrows, int
   buf[(size_t)i * nnn + j] = v;
 }
 
-__global__ void k_up_spread(const float2* __restrict__ buf, int nnn, float2* __restrict__ up) {
+// The half-sample delay of the spectrum: row i of buf times exp(j pi ks/nnn),
+// ks = k for k < nnn/2, k - nnn above, into row i of odd.
+__global__ void k_up_ramp(const float2* __restrict__ buf, int nnn, float2* __restrict__ odd) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   const int i = blockIdx.y;
-  const int n2 = 2 * nnn;
-  if (k >= n2) return;
-  float2 v = make_float2(0.f, 0.f);
-  if (k < nnn / 2)
-    v = buf[(size_t)i * nnn + k];
-  else if (k >= n2 - nnn / 2)
-    v = buf[(size_t)i * nnn + (k - nnn)];
-  up[(size_t)i * n2 + k] = v;
+  if (k >= nnn) return;
+  const int ks = k < nnn / 2 ? k : k - nnn;
+  float sn, cs;
+  sincospif((float)ks / (float)nnn, &sn, &cs);   // ks/nnn is exact in float
+  const float2 a = buf[(size_t)i * nnn + k];
+  odd[(size_t)i * nnn + k] = make_float2(a.x * cs - a.y * sn, a.x * sn + a.y * cs);
 }
 
-__device__ __forceinline__ float2 up_at(const float2* up, size_t idx, float fn) {
-  const float2 a = up[idx];
+// Even and odd samples 2j, 2j+1 of a row upsampled by two: x_j scale / 2 and
+// the odd transform divided by nnn and by 2.
+__device__ __forceinline__ float2 up_odd(const float2* odd, size_t idx, float fn) {
+  const float2 a = odd[idx];
   return make_float2((a.x / fn) / 2.0f, (a.y / fn) / 2.0f);
 }
 
 __global__ void k_xmul(const float2* __restrict__ up1, const float2* __restrict__ up2, int nnn,
                        const float2* __restrict__ img1, const float2* __restrict__ img2, int na,
-                       int row0, int nout_lines_chunk, int out_line0, int looksac, int looksdn,
-                       int mask, float2* __restrict__ igram, float2* __restrict__ amp) {
+                       float scale, int row0, int nout_lines_chunk, int out_line0, int looksac,
+                       int looksdn, int mask, float2* __restrict__ igram,
+                       float2* __restrict__ amp) {
   const int nw = na / looksac;
   const int oc = blockIdx.x * blockDim.x + threadIdx.x;
   const int ol = blockIdx.y;
   if (oc >= nw || ol >= nout_lines_chunk) return;
-  const int n2 = 2 * nnn;
   const float fn = (float)nnn;
   float ig_r = 0.f, ig_i = 0.f, a1 = 0.f, a2 = 0.f;
   float m1r = 0.f, m1i = 0.f, m2r = 0.f, m2i = 0.f;
@@ -84,9 +90,12 @@ __global__ void k_xmul(const float2* __restrict__ up1, const float2* __restrict_
     float s1r = 0.f, s1i = 0.f, s2r = 0.f, s2i = 0.f;
     for (int r = 0; r < looksdn; ++r) {
       const int il = ol * looksdn + r;                   // row in the batch
-      const size_t b = (size_t)il * n2 + 2 * j;
-      const float2 ua = up_at(up1, b, fn), ub = up_at(up1, b + 1, fn);
-      const float2 va = up_at(up2, b, fn), vb = up_at(up2, b + 1, fn);
+      const size_t o = (size_t)(row0 + il) * na + j;
+      const float2 x1 = img1[o], x2 = img2[o];
+      const float2 ua = make_float2(x1.x * scale * 0.5f, x1.y * scale * 0.5f);
+      const float2 va = make_float2(x2.x * scale * 0.5f, x2.y * scale * 0.5f);
+      const size_t b = (size_t)il * nnn + j;
+      const float2 ub = up_odd(up1, b, fn), vb = up_odd(up2, b, fn);
       const float pr = (ua.x * va.x + ua.y * va.y) + (ub.x * vb.x + ub.y * vb.y);
       const float pi = (ua.y * va.x - ua.x * va.y) + (ub.y * vb.x - ub.x * vb.y);
       const float ha = hypotf(ua.x, ua.y), hb = hypotf(ub.x, ub.y);
@@ -96,8 +105,6 @@ __global__ void k_xmul(const float2* __restrict__ up1, const float2* __restrict_
       c1 += ha * ha + hb * hb;
       c2 += ga * ga + gb * gb;
       if (mask) {
-        const size_t o = (size_t)(row0 + il) * na + j;
-        const float2 x1 = img1[o], x2 = img2[o];
         s1r += x1.x;
         s1i += x1.y;
         s2r += x2.x;
@@ -140,8 +147,8 @@ __global__ void k_makecc(const float2* __restrict__ ig, const float2* __restrict
 // transform length seen; a call uses its own length nnn (rows nnn apart).
 struct IgState {
   size_t nnn = 0;
-  float2* buf = nullptr;   // [kRows][nnn]
-  float2* up1 = nullptr;   // [kRows][2 nnn]
+  float2* buf = nullptr;   // [kRows][nnn] spectra
+  float2* up1 = nullptr;   // [kRows][nnn] odd samples of image 1
   float2* up2 = nullptr;
 };
 std::mutex g_ig_mu;
@@ -155,8 +162,8 @@ int buffers(IgState& s, size_t nnn) {
   if (s.up2) (void)hipFree(s.up2);
   s = IgState();
   if (hipMalloc((void**)&s.buf, sizeof(float2) * kRows * nnn) != hipSuccess ||
-      hipMalloc((void**)&s.up1, sizeof(float2) * kRows * 2 * nnn) != hipSuccess ||
-      hipMalloc((void**)&s.up2, sizeof(float2) * kRows * 2 * nnn) != hipSuccess)
+      hipMalloc((void**)&s.up1, sizeof(float2) * kRows * nnn) != hipSuccess ||
+      hipMalloc((void**)&s.up2, sizeof(float2) * kRows * nnn) != hipSuccess)
     return set_error(TDBP_ERR_NOMEM, "interferogram buffers (nnn %zu)", nnn);
   s.nnn = nnn;
   return TDBP_OK;
@@ -169,10 +176,10 @@ int upsample(const float2* img, int na, int rows, int row0, int nvalid_rows, siz
   int e = hip_check(hipGetLastError(), "upsample load");
   if (!e) e = fft_run(nnn, kRows, false, s.buf, st);
   if (e) return e;
-  hipLaunchKernelGGL(k_up_spread, dim3((unsigned)((2 * nnn + 255) / 256), kRows), dim3(256), 0,
-                     st, s.buf, (int)nnn, up);
-  e = hip_check(hipGetLastError(), "upsample spread");
-  if (!e) e = fft_run(2 * nnn, kRows, true, up, st);
+  hipLaunchKernelGGL(k_up_ramp, dim3((unsigned)((nnn + 255) / 256), kRows), dim3(256), 0, st,
+                     s.buf, (int)nnn, up);
+  e = hip_check(hipGetLastError(), "upsample ramp");
+  if (!e) e = fft_run(nnn, kRows, true, up, st);
   return e;
 }
 
@@ -210,7 +217,7 @@ int tdbp_crossmul(const void* d_img1, const void* d_img2, int na, int nvalid, fl
     const int rows = nol * looksdn;
     if ((st = upsample(i1, na, rows, row0, used_rows, nnn, scale, s, s.up1, str))) return st;
     if (i2 == i1) {
-      if ((st = hip_check(hipMemcpyAsync(s.up2, s.up1, sizeof(float2) * kRows * 2 * nnn,
+      if ((st = hip_check(hipMemcpyAsync(s.up2, s.up1, sizeof(float2) * kRows * nnn,
                                          hipMemcpyDeviceToDevice, str),
                           "upsample copy")))
         return st;
@@ -219,8 +226,8 @@ int tdbp_crossmul(const void* d_img1, const void* d_img2, int na, int nvalid, fl
     }
     const int nw = na / looksac;
     hipLaunchKernelGGL(k_xmul, dim3((unsigned)((nw + 127) / 128), nol), dim3(128), 0, str,
-                       s.up1, s.up2, (int)nnn, i1, i2, na, row0, nol, ol0, looksac, looksdn,
-                       mask ? 1 : 0, (float2*)d_igram, (float2*)d_amp);
+                       s.up1, s.up2, (int)nnn, i1, i2, na, scale, row0, nol, ol0, looksac,
+                       looksdn, mask ? 1 : 0, (float2*)d_igram, (float2*)d_amp);
     if ((st = hip_check(hipGetLastError(), "crossmul launch"))) return st;
   }
   return TDBP_OK;
