@@ -128,6 +128,17 @@ __device__ __forceinline__ void store_out(const BurstDev& b, const PixelCtx& p,
 
 // One pixel-pulse pair in FP64: range, gate, FP32 lerp of the two samples,
 // phase exp(+j 4 pi R / lambda) reduced in cycles before sincospi.
+// Global-address-space view of the sample rows: keeps the gather a
+// global_load (scalar base + 32-bit lane offset) even when the row pointer
+// comes from the burst table in memory.
+typedef const char __attribute__((address_space(1)))* grow_t;
+typedef uint32_t U32x2 __attribute__((ext_vector_type(2), aligned(4)));
+typedef float F32x4 __attribute__((ext_vector_type(4), aligned(8)));
+template <class T>
+__device__ __forceinline__ T gload(grow_t p) {
+  return *(const __attribute__((address_space(1))) T*)p;
+}
+
 // sqrt of a range squared (normal, ~1e12 m^2): the hardware reciprocal-root
 // estimate refined by goldschmidt_sqrt.
 __device__ __forceinline__ double range_sqrt(double x) {
@@ -147,7 +158,8 @@ __device__ __forceinline__ void plain_pair(const float2* __restrict__ row,
     int i = (int)fl;
     i = i > b.S - 2 ? b.S - 2 : i;
     const double fr = r - fl;
-    const float2 b0 = row[i], b1 = row[i + 1];
+    const F32x4 v = gload<F32x4>((grow_t)row + (unsigned)i * 8u);   // samples i, i+1
+    const float2 b0 = make_float2(v.x, v.y), b1 = make_float2(v.z, v.w);
     const float w0 = (float)(1.0 - fr), w1 = (float)fr;
     const float cr = b0.x * w0 + b1.x * w1;
     const float ci = b0.y * w0 + b1.y * w1;
@@ -155,8 +167,11 @@ __device__ __forceinline__ void plain_pair(const float2* __restrict__ row,
     const double f = cyc - rint(cyc);
     double sn, cs;
     sincos_2pi(f, &sn, &cs);
-    ar += (double)cr * cs - (double)ci * sn;
-    ai += (double)cr * sn + (double)ci * cs;
+    const double dr = cr, di = ci;
+    ar = fma(dr, cs, ar);
+    ar = fma(-di, sn, ar);
+    ai = fma(dr, sn, ai);
+    ai = fma(di, cs, ai);
   }
 }
 
@@ -244,16 +259,6 @@ struct TileShared {
   unsigned pmax;     // bits of max |sd| of the quadratics over their windows
 };
 
-// Global-address-space view of the sample rows: keeps the gather a
-// global_load (scalar base + 32-bit lane offset) even when the row pointer
-// comes from the burst table in memory.
-typedef const char __attribute__((address_space(1)))* grow_t;
-typedef uint32_t U32x2 __attribute__((ext_vector_type(2), aligned(4)));
-typedef float F32x4 __attribute__((ext_vector_type(4), aligned(8)));
-template <class T>
-__device__ __forceinline__ T gload(grow_t p) {
-  return *(const __attribute__((address_space(1))) T*)p;
-}
 
 // Structured-buffer view of a burst's samples (stride = one sample), so a
 // gather addresses sample vindex at voffset + soffset bytes with no address
