@@ -88,6 +88,9 @@ hipError_t launch_merge(float2* grid, const float2* out, size_t n, size_t grid_o
                         int semantics, hipStream_t s);
 hipError_t launch_pack(const float2* in, int L, int S, int dtype, void* out,
                        float* scale, hipStream_t s);
+// Common power-of-two scale for the pulses of one packed call (k_unify_scales).
+hipError_t launch_unify_scales(uint32_t* packed, size_t rowwords, int L, float* scale,
+                               int dtype, hipStream_t s);
 hipError_t launch_synth_clutter(float2* d, int L, int S, uint64_t seed,
                                 float sigma, const float* taps, int ntaps,
                                 hipStream_t s);

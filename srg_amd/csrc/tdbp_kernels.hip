@@ -345,6 +345,28 @@ __device__ __forceinline__ float2 lerp_raw(const typename RawPair<DT>::type& v, 
   }
 }
 
+// lerp_raw for a stage whose pulses share one scale, applied to the stage's
+// sum instead: u0 + fr (u1 - u0) as u0 - fr u0 + fr u1, two FMAs per
+// component on the raw words (FP16 through the mixed-precision FMA).
+template <int DT>
+__device__ __forceinline__ float2 lerp_raw_unscaled(const typename RawPair<DT>::type& v,
+                                                    float fr) {
+  if constexpr (DT == TDBP_CF32) {
+    return make_float2(fmaf(fr, v.z - v.x, v.x), fmaf(fr, v.w - v.y, v.y));
+  } else if constexpr (DT == TDBP_CF16) {
+    typedef _Float16 h2 __attribute__((ext_vector_type(2)));
+    const h2 u0 = __builtin_bit_cast(h2, (uint32_t)v.x);
+    const h2 u1 = __builtin_bit_cast(h2, (uint32_t)v.y);
+    return make_float2(fmaf(fr, (float)u1.x, fmaf(-fr, (float)u0.x, (float)u0.x)),
+                       fmaf(fr, (float)u1.y, fmaf(-fr, (float)u0.y, (float)u0.y)));
+  } else {
+    const float a0 = __uint_as_float(v.x << 16), a1 = __uint_as_float(v.y << 16);
+    const float b0 = __uint_as_float(v.x & 0xffff0000u),
+                b1 = __uint_as_float(v.y & 0xffff0000u);
+    return make_float2(fmaf(fr, a1, fmaf(-fr, a0, a0)), fmaf(fr, b1, fmaf(-fr, b0, b0)));
+  }
+}
+
 // Per-lane constants of the mixed pulse loop.  With d = x - x0, aref =
 // u_ref.d, bb = |d|^2 and w = 1 - aref/R0 + (aref/R0)^2 (1/(R0+a) ~ w/R0):
 // the range excess over R0 + aref is
@@ -381,10 +403,10 @@ struct PolyPixel {
 // by the sample index i0s + ia + floor(rho) = ibase + floor(rho)).  lim:
 // rho <= lim <=> r <= r_hi.
 template <int DT, bool CHECK>
-__device__ __forceinline__ void poly_pair(const PolyPixel& pp, const float4& r3, grow_t rowq,
-                                          grow_t rows, float Kf, float invd, bool inwin,
-                                          int ibase, unsigned smax, float lim, float& sr,
-                                          float& si) {
+__device__ __forceinline__ void poly_pair(const PolyPixel& pp, const float4& r3, float sc,
+                                          grow_t rowq, grow_t rows, float Kf, float invd,
+                                          bool inwin, int ibase, unsigned smax, float lim,
+                                          float& sr, float& si) {
   const float P = fmaf(fmaf(pp.c2, r3.x, pp.c1), r3.x, pp.c0);
   const float rho = fmaf(P, invd, r3.y);
   int il;
@@ -394,9 +416,9 @@ __device__ __forceinline__ void poly_pair(const PolyPixel& pp, const float4& r3,
   if constexpr (CHECK) {
     const int idx = il + ibase;
     if (!(inwin && (unsigned)idx <= smax && rho <= lim)) return;
-    c = lerp_pair<DT>(rowq, (unsigned)idx * (unsigned)elem_bytes<DT>(), fr, r3.w);
+    c = lerp_pair<DT>(rowq, (unsigned)idx * (unsigned)elem_bytes<DT>(), fr, sc);
   } else {
-    c = lerp_pair<DT>(rows, (unsigned)il * (unsigned)elem_bytes<DT>() + pp.iaoff, fr, r3.w);
+    c = lerp_pair<DT>(rows, (unsigned)il * (unsigned)elem_bytes<DT>() + pp.iaoff, fr, sc);
   }
   const float cyc = fmaf(Kf, P, r3.z);
   const float sn = __builtin_amdgcn_sinf(cyc), cs = __builtin_amdgcn_cosf(cyc);
@@ -484,14 +506,65 @@ __device__ __forceinline__ PairFront<DT> mixed_front(const MixedPixel& m, const 
   }
   return f;
 }
-template <int DT>
+// U: the stage's pulses share one scale, applied to the stage sum.
+template <int DT, bool U = false>
 __device__ __forceinline__ void mixed_back(const PairFront<DT>& f, float& sr, float& si) {
-  const float2 c = lerp_raw<DT>(f.v, f.fr, f.sc);
+  const float2 c = U ? lerp_raw_unscaled<DT>(f.v, f.fr) : lerp_raw<DT>(f.v, f.fr, f.sc);
   const float sn = __builtin_amdgcn_sinf(f.cyc), cs = __builtin_amdgcn_cosf(f.cyc);
   sr = fmaf(c.x, cs, sr);
   sr = fmaf(-c.y, sn, sr);
   si = fmaf(c.x, sn, si);
   si = fmaf(c.y, cs, si);
+}
+
+// The test-free core [qq, cb) of a stage on the quadratic form.  U: the
+// stage's pulses share one scale, which the caller applies to the stage sum.
+template <int DT, bool U>
+__device__ __forceinline__ void poly_core(const MixedPixel& m, const PolyPixel& pp,
+                                          const PulseRec* rec, int p0,
+                                          int qq, int cb, grow_t base, const I32x4& rsrc,
+                                          size_t rowbytes, long shift, int ibase, float Kf,
+                                          float invd, unsigned smax, float lim, float& sr,
+                                          float& si) {
+  // Three pairs in flight: the load of pulse q + 2 is issued before the
+  // back of pulse q.  (cb - ca) mod 3 plain pairs first; the rest is a
+  // multiple of 3, so a loop pass (taken when >= 6 remain) reads records
+  // q..q+4 inside the core.  Measured: 2.6 % above two in flight on the
+  // 27-burst IW scene (the quadratic's shorter body leaves less work to
+  // cover the gather with).
+  const int rem = (cb - qq) % 3;
+  for (int r = 0; r < rem; ++r, ++qq) {
+    const grow_t rowq = base + (size_t)qq * rowbytes;
+    const float4 r3 = rec[qq - p0].r3;
+    poly_pair<DT, false>(pp, r3, U ? 1.f : r3.w, rowq, rowq + shift, Kf, invd, true, ibase,
+                         smax, lim, sr, si);
+  }
+  if (qq < cb) {
+    grow_t rp = base + (size_t)qq * rowbytes;
+    unsigned so = (unsigned)((long)qq * (long)rowbytes + shift + kSbufBias);
+    const unsigned rb32 = (unsigned)rowbytes;
+    const PulseRec* rr = rec + (qq - p0);
+    PairFront<DT> fa = mixed_front<DT, true>(m, pp, rr[0], rp, rsrc, so, Kf, invd);
+    PairFront<DT> fb = mixed_front<DT, true>(m, pp, rr[1], rp, rsrc, so + rb32, Kf, invd);
+    PairFront<DT> fc;
+    for (; qq + 3 < cb; qq += 3) {
+      fc = mixed_front<DT, true>(m, pp, rr[2], rp, rsrc, so + 2 * rb32, Kf, invd);
+      __builtin_amdgcn_sched_barrier(0);
+      mixed_back<DT, U>(fa, sr, si);
+      fa = mixed_front<DT, true>(m, pp, rr[3], rp, rsrc, so + 3 * rb32, Kf, invd);
+      __builtin_amdgcn_sched_barrier(0);
+      mixed_back<DT, U>(fb, sr, si);
+      fb = mixed_front<DT, true>(m, pp, rr[4], rp, rsrc, so + 4 * rb32, Kf, invd);
+      __builtin_amdgcn_sched_barrier(0);
+      mixed_back<DT, U>(fc, sr, si);
+      so += 3 * rb32;
+      rr += 3;
+    }
+    fc = mixed_front<DT, true>(m, pp, rr[2], rp, rsrc, so + 2 * rb32, Kf, invd);
+    mixed_back<DT, U>(fa, sr, si);
+    mixed_back<DT, U>(fb, sr, si);
+    mixed_back<DT, U>(fc, sr, si);
+  }
 }
 
 // One stage's pulses [qa, qb) for one wave: checked edges [qa, ea) and
@@ -502,9 +575,9 @@ __device__ __forceinline__ void mixed_back(const PairFront<DT>& f, float& sr, fl
 template <int DT, bool POLY>
 __device__ __forceinline__ void stage_pulses(const MixedPixel& m, const PolyPixel& pp,
                                              const PulseRec* rec, int p0, int qa, int qb,
-                                             int qc0, int qc1, bool stage_safe, grow_t base,
-                                             const I32x4& rsrc, size_t rowbytes, long shift,
-                                             int ibase, float lim,
+                                             int qc0, int qc1, bool stage_safe, bool uni,
+                                             grow_t base, const I32x4& rsrc, size_t rowbytes,
+                                             long shift, int ibase, float lim,
                                              float Kf, float invd, int azs, unsigned n,
                                              unsigned smax, float& sr, float& si) {
   const int ea = min(qb, qc0);                      // [qa, ea): checked
@@ -513,7 +586,8 @@ __device__ __forceinline__ void stage_pulses(const MixedPixel& m, const PolyPixe
   auto checked = [&](int qq) {
     const grow_t rowq = base + (size_t)qq * rowbytes;
     if constexpr (POLY)
-      poly_pair<DT, true>(pp, rec[qq - p0].r3, rowq, rowq, Kf, invd, (unsigned)(qq - azs) < n,
+      poly_pair<DT, true>(pp, rec[qq - p0].r3, uni ? 1.f : rec[qq - p0].r3.w, rowq, rowq, Kf,
+                          invd, (unsigned)(qq - azs) < n,
                           ibase, smax, lim, sr, si);
     else
       mixed_pair<DT, true>(m, rec[qq - p0], rowq, Kf, invd, (unsigned)(qq - azs) < n, smax, sr,
@@ -522,7 +596,8 @@ __device__ __forceinline__ void stage_pulses(const MixedPixel& m, const PolyPixe
   auto unchecked = [&](int qq) {
     const grow_t rowq = base + (size_t)qq * rowbytes;
     if constexpr (POLY)
-      poly_pair<DT, false>(pp, rec[qq - p0].r3, rowq, rowq + shift, Kf, invd, true, ibase,
+      poly_pair<DT, false>(pp, rec[qq - p0].r3, uni ? 1.f : rec[qq - p0].r3.w, rowq,
+                           rowq + shift, Kf, invd, true, ibase,
                            smax, lim, sr, si);
     else
       mixed_pair<DT, false>(m, rec[qq - p0], rowq, Kf, invd, true, smax, sr, si);
@@ -534,40 +609,12 @@ __device__ __forceinline__ void stage_pulses(const MixedPixel& m, const PolyPixe
     // load there).  An empty core (cb <= ca) does nothing.
     int qq = ca;
     if constexpr (POLY) {
-      // Three pairs in flight: the load of pulse q + 2 is issued before the
-      // back of pulse q.  (cb - ca) mod 3 plain pairs first; the rest is a
-      // multiple of 3, so a loop pass (taken when >= 6 remain) reads records
-      // q..q+4 inside the core.  Measured: 2.6 % above two in flight on the
-      // 27-burst IW scene (the quadratic's shorter body leaves less work to
-      // cover the gather with).
-      const int rem = (cb - qq) % 3;
-      for (int r = 0; r < rem; ++r) unchecked(qq++);
-      if (qq < cb) {
-        grow_t rp = base + (size_t)qq * rowbytes;
-        unsigned so = (unsigned)((long)qq * (long)rowbytes + shift + kSbufBias);
-        const unsigned rb32 = (unsigned)rowbytes;
-        const PulseRec* rr = rec + (qq - p0);
-        PairFront<DT> fa = mixed_front<DT, POLY>(m, pp, rr[0], rp, rsrc, so, Kf, invd);
-        PairFront<DT> fb = mixed_front<DT, POLY>(m, pp, rr[1], rp, rsrc, so + rb32, Kf, invd);
-        PairFront<DT> fc;
-        for (; qq + 3 < cb; qq += 3) {
-          fc = mixed_front<DT, POLY>(m, pp, rr[2], rp, rsrc, so + 2 * rb32, Kf, invd);
-          __builtin_amdgcn_sched_barrier(0);
-          mixed_back<DT>(fa, sr, si);
-          fa = mixed_front<DT, POLY>(m, pp, rr[3], rp, rsrc, so + 3 * rb32, Kf, invd);
-          __builtin_amdgcn_sched_barrier(0);
-          mixed_back<DT>(fb, sr, si);
-          fb = mixed_front<DT, POLY>(m, pp, rr[4], rp, rsrc, so + 4 * rb32, Kf, invd);
-          __builtin_amdgcn_sched_barrier(0);
-          mixed_back<DT>(fc, sr, si);
-          so += 3 * rb32;
-          rr += 3;
-        }
-        fc = mixed_front<DT, POLY>(m, pp, rr[2], rp, rsrc, so + 2 * rb32, Kf, invd);
-        mixed_back<DT>(fa, sr, si);
-        mixed_back<DT>(fb, sr, si);
-        mixed_back<DT>(fc, sr, si);
-      }
+      if (uni)
+        poly_core<DT, true>(m, pp, rec, p0, qq, cb, base, rsrc, rowbytes, shift, ibase, Kf, invd,
+                            smax, lim, sr, si);
+      else
+        poly_core<DT, false>(m, pp, rec, p0, qq, cb, base, rsrc, rowbytes, shift, ibase, Kf, invd,
+                             smax, lim, sr, si);
     } else {
       // FP32 chain: two pairs in flight; an odd count starts with one plain
       // pair.  Measured: one pair in flight 1240, two 1442-1546, three 1% below
@@ -821,7 +868,11 @@ __device__ __forceinline__ void mixed_tile(const BurstDev& b, int tile,
   for (int p0 = wlo; p0 < whi; p0 += kStage) {
     // Stage prologue (FP64): one pulse per thread.
     const int qs = p0 + tid;
-    bool safe_all = true;
+    bool safe_all = true, uni = true;
+    // The scale of the stage's first pulse; the stage is "uniform" when every
+    // pulse carries it (the pack's common scale), and then the scale is
+    // applied to the stage sums rather than to each pair.
+    const float sc0 = (DT == TDBP_CF32 || b.scale == nullptr) ? 1.f : b.scale[p0];
     if (qs < whi) {
       const double* s = b.satloc + 3 * qs;
       const double u0 = x0[0] - s[0], u1 = x0[1] - s[1], u2 = x0[2] - s[2];
@@ -852,21 +903,31 @@ __device__ __forceinline__ void mixed_tile(const BurstDev& b, int tile,
       r.r3 = make_float4((float)(qs - pc), (float)f0s, cf, sc);
       rec[tid] = r;
       safe_all = safe;
+      uni = sc == sc0;
     }
     // Barrier that also tells whether every pulse of the stage is safe.
     const int stage_safe = __syncthreads_and(safe_all ? 1 : 0);
+    const bool stage_uni = DT == TDBP_CF32 || b.scale == nullptr ||
+                           __syncthreads_and(uni ? 1 : 0) != 0;
     float sr = 0.f, si = 0.f;
     // A wave without a live pixel skips the pulse loops (its window bounds
     // are sentinels and must not enter index arithmetic).
     if (wave_live) {
       const int qa = max(p0, qlo), qb = min(p0 + kStage, qhi);
       if (wave_poly)
-        stage_pulses<DT, true>(m, pp, rec, p0, qa, qb, qc0, qc1, stage_safe != 0, base, rsrc,
+        stage_pulses<DT, true>(m, pp, rec, p0, qa, qb, qc0, qc1, stage_safe != 0, stage_uni,
+                               base, rsrc,
                                rowbytes, (long)(i0s + ia_min) * elem_bytes<DT>(), i0s + ia,
                                (float)(b.r_hi - i0s - ia), Kf, invd, p.azs, n, smax, sr, si);
       else
-        stage_pulses<DT, false>(m, pp, rec, p0, qa, qb, qc0, qc1, stage_safe != 0, base, rsrc,
+        stage_pulses<DT, false>(m, pp, rec, p0, qa, qb, qc0, qc1, stage_safe != 0, false,
+                                base, rsrc,
                                 rowbytes, 0, 0, 0.f, Kf, invd, p.azs, n, smax, sr, si);
+    }
+    // A uniform stage summed the quadratic form's pairs unscaled.
+    if (DT != TDBP_CF32 && stage_uni && wave_poly) {
+      sr *= sc0;
+      si *= sc0;
     }
     // Kahan-compensated join of the stage sums.
     {
@@ -1682,13 +1743,59 @@ __global__ void k_pack(const float2* __restrict__ in, int S, int dtype,
   float mx = part[0];
   for (int w = 1; w < kBlock / kWave; ++w) mx = fmaxf(mx, part[w]);
   const float sc = pulse_scale(mx);
-  const float inv = 1.0f / sc;
+  const float inv = pulse_inv(sc);
   if (threadIdx.x == 0) scale[p] = sc;
   uint32_t* orow = out + (size_t)p * S;
   for (int i = threadIdx.x; i < S; i += blockDim.x) {
     const float2 v = row[i];
     orow[i] = pack_word(v.x * inv, v.y * inv, dtype);
   }
+}
+
+// One block per pulse, after k_pack or the fused range-compression store:
+// M = the largest scale of the L pulses; a pulse whose scale s is within
+// [M 2^-16, M) has its words multiplied by s/M (a power of two: exact unless
+// the FP16 result is subnormal) and takes M, an all-zero pulse (s = 0) takes
+// M, a pulse further below keeps its own scale.  Every block reads the same
+// M: the scales it rewrites become M, which leaves the maximum unchanged
+// (all-zero input: M = 0, every scale becomes 1).
+__global__ void k_unify_scales(uint32_t* __restrict__ packed, size_t rowwords, int L,
+                               float* __restrict__ scale, int dtype) {
+  __shared__ float part[kBlock / kWave];
+  const int p = blockIdx.x;
+  float m = 0.f;
+  for (int i = threadIdx.x; i < L; i += blockDim.x) m = fmaxf(m, scale[i]);
+  for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o, 64));
+  if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = m;
+  __syncthreads();
+  float M = part[0];
+  for (int w = 1; w < kBlock / kWave; ++w) M = fmaxf(M, part[w]);
+  const float s = scale[p];
+  __syncthreads();  // every thread has read scale[p] before it changes
+  if (M == 0.f) {
+    if (threadIdx.x == 0) scale[p] = 1.f;
+    return;
+  }
+  if (s == M || (s > 0.f && s < M * (1.0f / 65536.0f))) return;
+  if (s > 0.f) {
+    const float f = s / M;
+    uint32_t* row = packed + (size_t)p * rowwords;
+    for (size_t i = threadIdx.x; i < rowwords; i += blockDim.x) {
+      const uint32_t w = row[i];
+      float re, im;
+      if (dtype == TDBP_CF16) {
+        typedef _Float16 h2 __attribute__((ext_vector_type(2)));
+        const h2 h = __builtin_bit_cast(h2, w);
+        re = (float)h.x;
+        im = (float)h.y;
+      } else {
+        re = __uint_as_float(w << 16);
+        im = __uint_as_float(w & 0xffff0000u);
+      }
+      row[i] = pack_word(re * f, im * f, dtype);
+    }
+  }
+  if (threadIdx.x == 0) scale[p] = M;
 }
 
 // Counter-based normal pair (splitmix64 hash -> Box-Muller).
@@ -2070,6 +2177,15 @@ hipError_t launch_pack(const float2* in, int L, int S, int dtype, void* out,
                        float* scale, hipStream_t s) {
   hipLaunchKernelGGL(k_pack, dim3(L), dim3(kBlock), 0, s, in, S, dtype,
                      (uint32_t*)out, scale);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  return launch_unify_scales((uint32_t*)out, (size_t)S, L, scale, dtype, s);
+}
+
+hipError_t launch_unify_scales(uint32_t* packed, size_t rowwords, int L, float* scale,
+                               int dtype, hipStream_t s) {
+  hipLaunchKernelGGL(k_unify_scales, dim3(L), dim3(kBlock), 0, s, packed, rowwords, L, scale,
+                     dtype);
   return hipGetLastError();
 }
 

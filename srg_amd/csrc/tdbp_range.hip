@@ -142,7 +142,7 @@ k_rc_store(const float2* __restrict__ raw, int nlines, int stride, int base,
     float mx = part[0];
     for (int w = 1; w < kStoreBlock / 64; ++w) mx = fmaxf(mx, part[w]);
     const float sc = pulse_scale(mx);
-    const float inv = 1.0f / sc;
+    const float inv = pulse_inv(sc);
     if (threadIdx.x == 0) scale[l] = sc;
     uint32_t* po = packed + (size_t)l * (kHdr + nvalid);
     for (int k = threadIdx.x; k < kHdr + nvalid; k += kStoreBlock) {
@@ -299,6 +299,10 @@ int tdbp_range_compress(const void* d_raw, int nlines, int raw_stride, const tdb
                        dtype);
     if ((st = hip_check(hipGetLastError(), "store launch"))) return st;
   }
+  if (d_packed)
+    return hip_check(launch_unify_scales((uint32_t*)d_packed, (size_t)(kHdr + c->nvalid),
+                                         nlines, d_scale, dtype, s),
+                     "scale launch");
   return TDBP_OK;
 }
 

@@ -386,27 +386,88 @@ def _call_dropin(data, sc, demlength, fdout, fddem, firstlat, latlons):
 
 
 def test_pack_roundtrip(gpu):
+    """Per pulse a power-of-two scale with max |component| / scale in [8192,
+    16384); pulses within 2^16 of the largest scale then share it (an exact
+    rescale), an all-zero pulse takes it, pulses further below keep their own.
+    FP16 words equal a direct rounding of x / scale wherever that is normal."""
     rng = np.random.default_rng(2)
     L, S = 37, 1001
     x = ((rng.standard_normal((L, S)) + 1j * rng.standard_normal((L, S))) *
          np.exp(rng.uniform(-8, 8, (L, 1)))).astype(np.complex64)
     x[5] = 0
     d = T.DeviceArray.from_host(x)
+    pmax = np.maximum(np.abs(x.real), np.abs(x.imag)).max(1)
+    own = np.zeros(L)
+    nz = pmax > 0
+    own[nz] = np.ldexp(1.0, np.frexp(pmax[nz])[1] - 14)
+    M = own.max()
+    assert np.all(pmax[nz] / own[nz] >= 8192) and np.all(pmax[nz] / own[nz] < 16384)
+    want_sc = np.where(~nz | (own >= M * 2.0 ** -16), M, own)
+    assert 0 < np.sum(want_sc != M) < L - 1          # both kinds of pulse present
     for dt, eps in ((T.CF16, 2 ** -11), (T.CBF16, 2 ** -8)):
         p, s = T.pack(d, L, S, dt)
         raw = p.download()
         sc = s.download()
+        assert np.array_equal(sc.astype(np.float64), want_sc)
         if dt == T.CF16:
             re = (raw & 0xFFFF).astype(np.uint16).view(np.float16).astype(np.float32)
             im = (raw >> 16).astype(np.uint16).view(np.float16).astype(np.float32)
+            for part, comp in ((re, x.real), (im, x.imag)):
+                q = comp / sc[:, None].astype(np.float32)
+                normal = np.abs(q) >= 2.0 ** -14
+                assert np.array_equal(part[normal], q[normal].astype(np.float16)
+                                      .astype(np.float32))
         else:
             re = ((raw & 0xFFFF) << 16).astype(np.uint32).view(np.float32)
             im = (raw & 0xFFFF0000).astype(np.uint32).view(np.float32)
         y = (re + 1j * im) * sc[:, None]
-        pmax = np.maximum(np.abs(x.real), np.abs(x.imag)).max(1, keepdims=True)
         err = np.maximum(np.abs(y.real - x.real), np.abs(y.imag - x.imag))
-        assert np.all(err <= eps * pmax + 1e-30)
-        assert sc[5] == 1.0 and np.all(y[5] == 0)
+        assert np.all(err <= eps * pmax[:, None] + 1e-30)
+        assert np.all(y[5] == 0)
+
+
+def _pack_host(x, scales):
+    """FP16 words of x / scale per pulse (nearest even), and the samples the
+    kernel reads from them (words times scale, exact in complex64)."""
+    s32 = scales.astype(np.float32)[:, None]
+    re = (x.real / s32).astype(np.float16)
+    im = (x.imag / s32).astype(np.float16)
+    words = re.view(np.uint16).astype(np.uint32) | (im.view(np.uint16).astype(np.uint32) << 16)
+    deq = (re.astype(np.float32) * s32) + 1j * (im.astype(np.float32) * s32)
+    return words, deq.astype(np.complex64)
+
+
+@pytest.mark.parametrize("kind", ["one", "two", "each"])
+def test_mixed_per_pulse_scales(c1, oracle, kind):
+    """FP16 samples whose per-pulse scales are all equal (stages apply the
+    scale to their sums), take two values (stages across the change use the
+    per-pair scale) or differ on every pulse, against the oracle on the
+    dequantized samples."""
+    d, _ = c1
+    sc = d.sc
+    x = d.data_h
+    pmax = np.maximum(np.abs(x.real), np.abs(x.imag)).max(1).astype(np.float64)
+    base = np.maximum(pmax, 1e-30) / 16384.0
+    if kind == "one":
+        scales = np.full(sc.L, 0.37 * base.max())
+    elif kind == "two":
+        scales = np.where(np.arange(sc.L) < sc.L // 2 + 7, 1.3 * base.max(), 0.8 * base.max())
+    else:
+        scales = base * np.random.default_rng(9).uniform(1.0, 1.9, sc.L)
+    words, deq = _pack_host(x, scales)
+    out, _, _, _, azoff = oracle.backproject(sc.geom, deq, sc.satloc, sc.dem)
+    d.fit()
+    smp = T.samples(T.DeviceArray.from_host(words), T.CF16,
+                    T.DeviceArray.from_host(scales.astype(np.float32)))
+    d.d_out.zero()
+    T.backproject(sc.geom, smp, d.d_sat, d.d_dem, d.d_out, T.MIXED, d_coef=d.d_coef)
+    T.sync()
+    got = d.d_out.download()
+    m = azoff > 0
+    ph, amp = errors(got, out, m)
+    print(f"per-pulse scales ({kind}): phase {ph*1e3:.4f} mrad, amp {amp:.3e}")
+    assert ph <= 1e-4 and amp <= 1e-5
+    assert np.all(got[~m] == 0)
 
 
 def test_howmanygpus_counts_the_gpu(gpu):
