@@ -231,7 +231,12 @@ typedef struct tdbp_samples {
 } tdbp_samples;
 
 /* Convert cf32 samples to CF16/CBF16 with a per-pulse FP32 scale
- * (scale = max |component| of the pulse / 2^14, 1 for an all-zero pulse). */
+ * (value = stored * scale).  Each pulse first gets the power of two with
+ * max |component| / scale in [2^13, 2^14); then every pulse whose scale is
+ * within 2^16 of the call's largest M, and every all-zero pulse, takes M (its
+ * words rescaled exactly by a power of two; all zero input: scale 1).  A
+ * burst's pulses thus normally share one scale, which the backprojection
+ * applies per stage of pulses; any per-pulse scales are accepted as input. */
 int tdbp_pack_samples(const void *d_cf32, int L, int S, int dtype,
                       void *d_packed, float *d_scale, void *stream);
 
@@ -442,7 +447,7 @@ int tdbp_chirp_decode(const uint8_t *header80, tdbp_chirp *c);
  *   d_packed  CF16/CBF16 words [nlines][nvalid + 10] and d_scale [nlines]:
  *             the backprojection input (header samples zero, as
  *             sentinel/backprojectgpusub.f90:190 leaves them) in the storage
- *             of tdbp_pack_samples.
+ *             of tdbp_pack_samples (scales made common over the nlines).
  * The values are the reference's: 32768-point forward FFT of the zero-padded
  * echo, times conj(FFT(reference chirp / npts)), inverse FFT without
  * normalisation, in FP32.  The kept samples are a linear correlation of the
