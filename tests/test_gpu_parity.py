@@ -385,13 +385,13 @@ def _call_dropin(data, sc, demlength, fdout, fddem, firstlat, latlons):
                                 latlons).call(fdout, fddem)
 
 
-def test_pack_roundtrip(gpu):
+@pytest.mark.parametrize("L,S", [(37, 1001), (1100, 64)])
+def test_pack_roundtrip(gpu, L, S):
     """Per pulse a power-of-two scale with max |component| / scale in [8192,
     16384); pulses within 2^16 of the largest scale then share it (an exact
     rescale), an all-zero pulse takes it, pulses further below keep their own.
     FP16 words equal a direct rounding of x / scale wherever that is normal."""
     rng = np.random.default_rng(2)
-    L, S = 37, 1001
     x = ((rng.standard_normal((L, S)) + 1j * rng.standard_normal((L, S))) *
          np.exp(rng.uniform(-8, 8, (L, 1)))).astype(np.complex64)
     x[5] = 0
