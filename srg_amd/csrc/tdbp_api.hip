@@ -211,7 +211,7 @@ int default_variant(int precision) {
   if (g_variant >= 0) return g_variant;
   const char* e = std::getenv("TDBP_VARIANT");
   const int v = e ? std::atoi(e) : 5;
-  return (v >= 0 && v < kNumVariants) ? v : 4;
+  return (v >= 0 && v < kNumVariants) ? v : 5;
 }
 
 // Orbit upload + fit on a stream with stream-ordered temporary memory.
@@ -603,27 +603,27 @@ int tdbp_synth_raw(void* d_raw, int L, int stride, const tdbp_chirp* c,
   const size_t nd = (size_t)L * (ntargets > 0 ? ntargets : 1);
   double *dd = nullptr, *dc = nullptr;
   float2* da = nullptr;
-  if ((st = hip_check(hipMallocAsync((void**)&dd, nd * 8, s), "synth_raw scratch")) ||
-      (st = hip_check(hipMallocAsync((void**)&dc, nd * 8, s), "synth_raw scratch")) ||
-      (st = hip_check(hipMallocAsync((void**)&da, 8 * (ntargets > 0 ? ntargets : 1), s),
-                      "synth_raw scratch")))
-    return st;
-  if (ntargets > 0) {
-    if ((st = hip_check(hipMemcpyAsync(dd, delay, nd * 8, hipMemcpyHostToDevice, s),
-                        "synth_raw upload")) ||
-        (st = hip_check(hipMemcpyAsync(dc, carrier, nd * 8, hipMemcpyHostToDevice, s),
-                        "synth_raw upload")) ||
-        (st = hip_check(hipMemcpyAsync(da, amp, 8 * (size_t)ntargets, hipMemcpyHostToDevice, s),
-                        "synth_raw upload")))
-      return st;
+  if (!(st = hip_check(hipMallocAsync((void**)&dd, nd * 8, s), "synth_raw scratch")) &&
+      !(st = hip_check(hipMallocAsync((void**)&dc, nd * 8, s), "synth_raw scratch")) &&
+      !(st = hip_check(hipMallocAsync((void**)&da, 8 * (ntargets > 0 ? ntargets : 1), s),
+                       "synth_raw scratch")) &&
+      ntargets > 0) {
+    if (!(st = hip_check(hipMemcpyAsync(dd, delay, nd * 8, hipMemcpyHostToDevice, s),
+                         "synth_raw upload")) &&
+        !(st = hip_check(hipMemcpyAsync(dc, carrier, nd * 8, hipMemcpyHostToDevice, s),
+                         "synth_raw upload")))
+      st = hip_check(hipMemcpyAsync(da, amp, 8 * (size_t)ntargets, hipMemcpyHostToDevice, s),
+                     "synth_raw upload");
   }
-  st = hip_check(launch_synth_raw((float2*)d_raw, L, stride, c->nsamps, c->npts, c->samplefreq,
-                                  c->startfreq, c->ramprate, dd, dc, da, ntargets, noise, seed,
-                                  header80, s),
-                 "synth_raw launch");
-  (void)hipFreeAsync(dd, s);
-  (void)hipFreeAsync(dc, s);
-  (void)hipFreeAsync(da, s);
+  if (!st)
+    st = hip_check(launch_synth_raw((float2*)d_raw, L, stride, c->nsamps, c->npts, c->samplefreq,
+                                    c->startfreq, c->ramprate, dd, dc, da, ntargets, noise, seed,
+                                    header80, s),
+                   "synth_raw launch");
+  // Freed on every path (a null pointer is a no-op).
+  if (dd) (void)hipFreeAsync(dd, s);
+  if (dc) (void)hipFreeAsync(dc, s);
+  if (da) (void)hipFreeAsync(da, s);
   if (!st) st = hip_check(hipStreamSynchronize(s), "synth_raw sync");
   return st;
 }

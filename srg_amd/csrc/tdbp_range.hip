@@ -37,7 +37,6 @@ constexpr int kFftLen = 32768;   // processsub.f90:39, the largest length used
 constexpr int kChunk = 256;      // lines per batched transform
 constexpr int kHdr = 10;         // header samples (80 bytes) per line
 constexpr double kFref = 37.53472224;
-constexpr int kMaxDev = 64;
 
 // ------------------------------------------------------------ header fields
 
