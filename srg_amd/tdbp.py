@@ -487,6 +487,11 @@ def pixelint(g: BurstGeometry, d_burst, d_satloc, d_xyz, d_azoff, d_out, stream=
 
 
 def pack(d_cf32: DeviceArray, L: int, S: int, dtype: int, stream=None):
+    """cf32 samples [L][S] -> (CF16/CBF16 words [L][S], FP32 scales [L])
+    (tdbp_pack_samples): power-of-two scales, one common scale for the pulses
+    within 2^16 of the largest, value = word x scale."""
+    if d_cf32.nbytes < L * S * 8:
+        raise ValueError("pack: input smaller than L x S cf32 samples")
     d_packed = DeviceArray((L, S), np.uint32)
     d_scale = DeviceArray((L,), np.float32)
     check(lib().tdbp_pack_samples(d_cf32.ptr, L, S, dtype, d_packed.ptr, d_scale.ptr,
