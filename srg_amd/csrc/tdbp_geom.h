@@ -161,10 +161,14 @@ TDBP_HD void dem_to_ecef(int line, int firstline, int pixel, double firstlat,
   const double lat = firstlat + (line + firstline) * deltalat;
   const double phi = lat * kDeg2Rad;
   const double lam = (firstlon + pixel * deltalon) * kDeg2Rad;
-  const double sphi = sin(phi), cphi = cos(phi);
+  // sincos: the library's sin and cos values from one shared argument
+  // reduction (the compiler does not merge separate sin and cos calls).
+  double sphi, cphi, slam, clam;
+  sincos(phi, &sphi, &cphi);
+  sincos(lam, &slam, &clam);
   const double re = kWgs84A / sqrt(1.0 - kWgs84E2 * sphi * sphi);
-  o[0] = (re + hgt) * cphi * cos(lam);
-  o[1] = (re + hgt) * cphi * sin(lam);
+  o[0] = (re + hgt) * cphi * clam;
+  o[1] = (re + hgt) * cphi * slam;
   o[2] = (re - (re * kWgs84E2) + hgt) * sphi;
 }
 
